@@ -1,0 +1,127 @@
+/*
+ * vsvb_hip.h -- C ABI of the MI355X VSVB energy path.
+ *
+ * The library replaces the body of VALENCE's energy kernel
+ *   subroutine vsvb_energy(iorb, num_non_docc, num_spatial_orbs, energy, wfnorm,
+ *                          spinav, ijkl_symmetry_is_enabled)      src/valence.F90:1010-1011
+ * as called from guess_energy (src/valence.F90:341), so that the public entry
+ * points valence_api_initialize / valence_api_calculate_energy /
+ * valence_api_finalize (src/valence_api.F90:9, 37, 114) keep their signatures.
+ * A Fortran host binds these functions with ISO_C_BINDING interfaces; every
+ * argument is a plain pointer or scalar, arrays are host memory unless the name
+ * says "device", and no GPU type appears in a signature except the opaque
+ * stream handle of the *_device entry points.
+ *
+ * Error behaviour: the reference has no error codes (it calls xm_abort).  Every
+ * function here returns 0 on success and a negative code otherwise;
+ * vsvb_last_error() gives the message (thread-local).  A Fortran shim calls
+ * xm_abort on a non-zero return.
+ */
+#ifndef VSVB_HIP_H
+#define VSVB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VSVB_OK 0
+#define VSVB_ERR_ARG (-1)      /* invalid argument / unsupported input */
+#define VSVB_ERR_DEVICE (-2)   /* HIP runtime or kernel error */
+#define VSVB_ERR_NUMERIC (-3)  /* singular overlap block, non-finite result */
+#define VSVB_ERR_UNSUPPORTED (-4)
+
+/*
+ * The wave function, molecule and basis as the reference holds them after
+ * xm_input (src/xm_module.F90:76-335; module integrals, src/integrals_module.F90;
+ * module densitywork, src/density_module.F90).  Indices are 0-based except
+ * xpset and pair_sc, which keep the reference's 1-based meaning.  Coordinates
+ * are in bohr (after angs2bohr, src/tools_module.F90:6-17).  Arrays are read
+ * during the call only; the library keeps copies.
+ */
+typedef struct vsvb_system {
+    int32_t natom, ntype, nshell_type, nprim, norbs, mxctr, totlen;
+    int32_t npair, nunpd, ndocc, ndf, nspinc;
+    int32_t ntol_d, ntol_i;             /* dtol = 10^-ntol_d, itol = 10^-ntol_i */
+    const int32_t *atom_type;           /* [natom]            atom_t - 1          */
+    const double *coords;               /* [natom][3] bohr    coords              */
+    const double *nuc_charge;           /* [ntype]            nuc_charge          */
+    const int32_t *type_shell_start;    /* [ntype+1]          map_atom2shell - 1  */
+    const int32_t *shell_l;             /* [nshell_type]      ang_mom             */
+    const int32_t *shell_prim_start;    /* [nshell_type+1]    map_shell2prim - 1  */
+    const double *prim_exp;             /* [nprim]            exponent            */
+    const double *prim_coef;            /* [nprim]            unnormalized_con_coeff */
+    const int32_t *orb_natom;           /* [norbs]            orbas_atnum         */
+    const int32_t *orb_atoms;           /* [norbs][mxctr]     orbas_atset - 1     */
+    const int32_t *orb_start;           /* [norbs+1]          map_orbs - 1        */
+    const int32_t *xpset;               /* [totlen]           xpset (1-based; <1 = NDF) */
+    const double *coeff;                /* [totlen]           coeff               */
+    const int32_t *pair_sc;             /* [max(nspinc,1)][npair][2] pair_sc (1-based slots) */
+    const double *coeff_sc;             /* [max(nspinc,1)]    coeff_sc            */
+} vsvb_system;
+
+typedef struct vsvb_options {
+    double quartet_tau;  /* AO shell-quartet Schwarz cut: skip if Q_ab*Q_cd < tau (<=0: default 1e-17) */
+    int32_t device;      /* HIP device ordinal */
+    int32_t reserved;
+} vsvb_options;
+
+typedef struct vsvb_context vsvb_context;
+
+/* Creates the device-resident state for one molecule/basis/orbital layout
+ * (called where valence_initialize reads the input, src/valence_initialize_module.F90:4-30).
+ * All device memory the energy calls need is allocated here. */
+int vsvb_context_create(const vsvb_system *sys, const vsvb_options *opt, vsvb_context **ctx);
+
+/* Frees everything (valence_finalize, src/valence_finalize_module.F90:3-21). */
+void vsvb_context_destroy(vsvb_context *ctx);
+
+/* Plain energy: E = (E1 + E2)/N + E_nuc as guess_energy returns it
+ * (src/valence.F90:309-345).  coords (bohr, [natom][3]) and coeff ([totlen])
+ * may be NULL to reuse the previous call's values (initially the ones in the
+ * vsvb_system).  parts, if not NULL, receives {E_el, N, E1, E_nuc} where
+ * E_el = (E1+E2)/N; the library returns N = 1 (numerator and norm are
+ * defined up to a common factor and it reports them normalized). */
+int vsvb_energy(vsvb_context *ctx, const double *coords, const double *coeff, double *energy, double *parts);
+
+/* The internal seam itself: replaces vsvb_energy's output pair (energy, wfnorm)
+ * (src/valence.F90:1010-1011, contract at :342-344: caller computes
+ * energy/wfnorm + enucrep).  Under MPI, call it on one rank and pass zeros on
+ * the others so that xm_equalize_scalar sums to the same values. */
+int vsvb_energy_parts(vsvb_context *ctx, const double *coeff, double *energy_numerator, double *wfnorm);
+
+/* Energy and its gradient with respect to every LCAO weight coeff[k]
+ * (adjoint pass over the same screened quartets, integrals recomputed).
+ * grad has totlen entries; d E / d coeff[k] at the given (unnormalized) weights. */
+int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coeff, double *energy, double *grad);
+
+/* Same as vsvb_energy_grad with device pointers on a caller stream
+ * (hipStream_t passed as void*).  coeff/energy/grad are device memory. */
+int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
+                            double *grad_device, void *stream);
+
+/* Batched evaluation: many wave functions of uneven size in one graph launch. */
+typedef struct vsvb_batch vsvb_batch;
+
+int vsvb_batch_create(int32_t nwfn, const vsvb_system *const *systems, const vsvb_options *opt, vsvb_batch **batch);
+/* energies: [nwfn]; grads: NULL or [nwfn] host pointers of totlen doubles each (NULL entries skipped). */
+int vsvb_batch_run(vsvb_batch *batch, int32_t want_grad, double *energies, double *const *grads);
+void vsvb_batch_destroy(vsvb_batch *batch);
+
+/* Diagnostics. */
+const char *vsvb_last_error(void);
+/* Counters of the last call: [0] shell quartets evaluated, [1] primitive quartets,
+ * [2] shell quartets skipped by the Schwarz cut, [3] device milliseconds of the ERI stage. */
+int vsvb_last_counters(vsvb_context *ctx, double *out4);
+/* Copy n doubles of a named internal device buffer to host memory (diagnostics and
+ * tests): "S_ao", "h_ao", "C", "C_eri", "S_orb", "h_orb", "P", "R", "J", "K", "F",
+ * "gradC", "scalars".  Layouts are row-major; see the implementation for shapes. */
+int vsvb_debug_buffer(vsvb_context *ctx, const char *name, double *out, int64_t n);
+/* Library version string. */
+const char *vsvb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VSVB_HIP_H */

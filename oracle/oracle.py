@@ -33,7 +33,7 @@ class OrcSystem(C.Structure):
 
 class OrcOptions(C.Structure):
     _fields_ = [("itol", C.c_double), ("dtol", C.c_double), ("nthreads", C.c_int), ("eri_table", C.c_int),
-                ("task_begin", C.c_longlong), ("task_end", C.c_longlong), ("skip_1e", C.c_int)]
+                ("task_begin", C.c_longlong), ("task_end", C.c_longlong), ("skip_1e", C.c_int), ("plant", C.c_int)]
 
 
 class OrcResult(C.Structure):
@@ -111,12 +111,13 @@ class OracleEnergy:
 
 
 def energy(s, itol: Optional[float] = None, dtol: Optional[float] = None, nthreads: int = 0,
-           eri_table: bool = True, task_begin: int = 0, task_end: int = -1, skip_1e: bool = False) -> OracleEnergy:
+           eri_table: bool = True, task_begin: int = 0, task_end: int = -1, skip_1e: bool = False,
+           plant: int = 0) -> OracleEnergy:
     """Plain VSVB energy of ``s`` (a VsvbSystem) by the restated reference algorithm."""
     keep = _Keep(s)
     opt = OrcOptions(itol=-1.0 if itol is None else itol, dtol=-1.0 if dtol is None else dtol,
                      nthreads=nthreads, eri_table=int(eri_table), task_begin=task_begin, task_end=task_end,
-                     skip_1e=int(skip_1e))
+                     skip_1e=int(skip_1e), plant=plant)
     res = OrcResult()
     rc = lib().orc_energy(C.byref(keep.cs), C.byref(opt), C.byref(res))
     if rc != 0:

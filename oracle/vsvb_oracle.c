@@ -1175,6 +1175,8 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
                 }
             } else
                 ++npo;
+            if (opt->plant == 1) xsum = 0.0;
+            if (opt->plant == 2) xsum = -xsum;
             if (!(ko == io && lo == jo)) { esum *= 2.0; xsum *= 2.0; }
             energy2 += esum - xsum;
         }

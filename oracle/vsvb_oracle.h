@@ -50,6 +50,7 @@ typedef struct {
     long long task_begin; /* 2e task slice [begin, end), 0-based; end < 0 = all */
     long long task_end;
     int skip_1e;          /* 1: skip the one-electron loop (task-slice timing) */
+    int plant;            /* tests only: 1 = drop the exchange term, 2 = flip its sign */
 } orc_options;
 
 typedef struct {

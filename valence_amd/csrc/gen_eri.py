@@ -1,0 +1,353 @@
+#!/usr/bin/env python3
+"""Generate straight-line FP64 ERI device code for every (s,p,d) shell-quartet class.
+
+Method (written from the published recurrences, not from any library):
+
+* Obara-Saika vertical recurrence (VRR) on primitive quartets builds
+  [e0|f0]^(0) for |e| in [la, la+lb] and |f| in [lc, lc+ld] from Boys
+  values F_m(T), m = 0..L:
+
+    [e+1i,0|00]^(m) = PA_i [e0|00]^(m) + WP_i [e0|00]^(m+1)
+                      + e_i/(2p) ([e-1i,0|00]^(m) - rho/p [e-1i,0|00]^(m+1))
+    [e0|f+1i,0]^(m) = QC_i [e0|f0]^(m) + WQ_i [e0|f0]^(m+1)
+                      + f_i/(2q) ([e0|f-1i,0]^(m) - rho/q [e0|f-1i,0]^(m+1))
+                      + e_i/(2(p+q)) [e-1i,0|f0]^(m+1)
+
+* The [e0|f0] are contracted over all primitive quartets (accumulators).
+* Head-Gordon-Pople horizontal recurrence (HRR) on contracted integrals:
+
+    (a, b+1i| = (a+1i, b| + AB_i (a, b|        |c, d+1i) = |c+1i, d) + CD_i |c, d)
+
+Every class gets one device function
+
+    eri_<la><lb><lc><ld>(bra prim pairs, ket prim pairs, AB, CD, boys table, out[])
+
+writing the contracted cartesian block in CCA order with the A index slowest
+(out[((a*nB + b)*nC + c)*nD + d]).  Classes are canonical: la >= lb, lc >= ld and
+the bra pair class is not below the ket pair class (pair class order below).
+Run as a script it writes ``eri_gen.inc`` next to itself and prints the FP64
+operation count per class (used for the algorithmic FLOP model).
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+from functools import lru_cache
+from typing import Dict, List, Tuple
+
+LMAX = 2
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]  # order used everywhere
+
+
+def cart(l: int) -> List[Tuple[int, int, int]]:
+    """CCA order (valence.F90:2364-2378): i=0..l, j=0..i -> (l-i, i-j, j)."""
+    out = []
+    for i in range(l + 1):
+        for j in range(i + 1):
+            out.append((l - i, i - j, j))
+    return out
+
+
+def quartet_classes() -> List[Tuple[int, int, int, int]]:
+    out = []
+    for bi, (la, lb) in enumerate(PAIR_CLASSES):
+        for ki in range(bi + 1):
+            lc, ld = PAIR_CLASSES[ki]
+            out.append((la, lb, lc, ld))
+    return out
+
+
+def sub(v, i, k=1):
+    w = list(v)
+    w[i] -= k
+    return tuple(w)
+
+
+def add(v, i, k=1):
+    w = list(v)
+    w[i] += k
+    return tuple(w)
+
+
+def name3(v):
+    return f"{v[0]}{v[1]}{v[2]}"
+
+
+class Emitter:
+    def __init__(self):
+        self.lines: List[str] = []
+        self.flops = 0
+
+    def emit(self, s: str):
+        self.lines.append(s)
+
+
+def gen_vrr(la, lb, lc, ld, em: Emitter):
+    """Emit VRR for one primitive quartet; returns dict target->(var name)."""
+    L = la + lb + lc + ld
+    targets = [(e, f) for le in range(la, la + lb + 1) for e in cart(le)
+               for lf in range(lc, lc + ld + 1) for f in cart(lf)]
+    order: List[Tuple] = []
+    recipe: Dict[Tuple, Tuple] = {}
+    seen = set()
+
+    def need(e, f, m):
+        if min(e) < 0 or min(f) < 0:
+            return None
+        key = (e, f, m)
+        if key in seen:
+            return key
+        # choose the decrement direction greedily: fewest new intermediates
+        if sum(f) > 0:
+            best = None
+            for i in range(3):
+                if f[i] == 0:
+                    continue
+                f1 = sub(f, i)
+                deps = [(e, f1, m), (e, f1, m + 1)]
+                if f1[i] > 0:
+                    deps += [(e, sub(f1, i), m), (e, sub(f1, i), m + 1)]
+                if e[i] > 0:
+                    deps += [(sub(e, i), f1, m + 1)]
+                new = sum(1 for d in deps if d not in seen)
+                if best is None or new < best[0]:
+                    best = (new, i)
+            i = best[1]
+            f1 = sub(f, i)
+            parts = [("QC", i, need(e, f1, m)), ("WQ", i, need(e, f1, m + 1))]
+            if f1[i] > 0:
+                parts.append(("F2Q", i, f1[i], need(e, sub(f1, i), m), need(e, sub(f1, i), m + 1)))
+            if e[i] > 0:
+                parts.append(("E2S", i, e[i], need(sub(e, i), f1, m + 1)))
+            recipe[key] = ("ket", parts)
+        elif sum(e) > 0:
+            best = None
+            for i in range(3):
+                if e[i] == 0:
+                    continue
+                e1 = sub(e, i)
+                deps = [(e1, f, m), (e1, f, m + 1)]
+                if e1[i] > 0:
+                    deps += [(sub(e1, i), f, m), (sub(e1, i), f, m + 1)]
+                new = sum(1 for d in deps if d not in seen)
+                if best is None or new < best[0]:
+                    best = (new, i)
+            i = best[1]
+            e1 = sub(e, i)
+            parts = [("PA", i, need(e1, f, m)), ("WP", i, need(e1, f, m + 1))]
+            if e1[i] > 0:
+                parts.append(("E2P", i, e1[i], need(sub(e1, i), f, m), need(sub(e1, i), f, m + 1)))
+            recipe[key] = ("bra", parts)
+        else:
+            recipe[key] = ("base", m)
+        seen.add(key)
+        order.append(key)
+        return key
+
+    for (e, f) in targets:
+        need(e, f, 0)
+
+    def var(k):
+        e, f, m = k
+        return f"v{name3(e)}_{name3(f)}_{m}"
+
+    xyz = "xyz"
+    for k in order:
+        r = recipe[k]
+        if r[0] == "base":
+            em.emit(f"    const double {var(k)} = F{r[1]};")
+            continue
+        terms = []
+        fl = 0
+        if r[0] == "bra":
+            for part in r[1]:
+                if part[0] == "PA":
+                    terms.append(f"PA{xyz[part[1]]} * {var(part[2])}"); fl += 2
+                elif part[0] == "WP":
+                    terms.append(f"WP{xyz[part[1]]} * {var(part[2])}"); fl += 2
+                elif part[0] == "E2P":
+                    n = part[2]
+                    terms.append(f"{float(n)!r} * oo2p * ({var(part[3])} - rop * {var(part[4])})"); fl += 5
+        else:
+            for part in r[1]:
+                if part[0] == "QC":
+                    terms.append(f"QC{xyz[part[1]]} * {var(part[2])}"); fl += 2
+                elif part[0] == "WQ":
+                    terms.append(f"WQ{xyz[part[1]]} * {var(part[2])}"); fl += 2
+                elif part[0] == "F2Q":
+                    n = part[2]
+                    terms.append(f"{float(n)!r} * oo2q * ({var(part[3])} - roq * {var(part[4])})"); fl += 5
+                elif part[0] == "E2S":
+                    n = part[2]
+                    terms.append(f"{float(n)!r} * oo2s * {var(part[3])}"); fl += 3
+        em.flops += fl - 1
+        em.emit(f"    const double {var(k)} = " + " + ".join(terms) + ";")
+    return {(e, f): var((e, f, 0)) for (e, f) in targets}, len(order), targets
+
+
+def gen_hrr(la, lb, lc, ld, accname, em: Emitter):
+    """Emit bra then ket HRR on contracted accumulators; returns out index map."""
+    fs = [f for lf in range(lc, lc + ld + 1) for f in cart(lf)]
+    # bra: (a,b|f) for |a|=la, |b|=lb
+    seen = {}
+    order = []
+
+    def bra(a, b):
+        key = (a, b)
+        if key in seen:
+            return key
+        if sum(b) == 0:
+            seen[key] = ("base",)
+        else:
+            i = next(i for i in range(3) if b[i] > 0)
+            b1 = sub(b, i)
+            seen[key] = ("rec", i, bra(add(a, i), b1), bra(a, b1))
+        order.append(key)
+        return key
+
+    finals_ab = [(a, b) for a in cart(la) for b in cart(lb)]
+    for a, b in finals_ab:
+        bra(a, b)
+
+    def bvar(key, f):
+        a, b = key
+        if sum(b) == 0:
+            return accname[(a, f)]
+        return f"hb{name3(a)}_{name3(b)}_{name3(f)}"
+
+    xyz = "xyz"
+    for key in order:
+        r = seen[key]
+        if r[0] == "base":
+            continue
+        for f in fs:
+            em.emit(f"    const double {bvar(key, f)} = {bvar(r[2], f)} + AB{xyz[r[1]]} * {bvar(r[3], f)};")
+            em.flops += 2
+    # ket: (ab|c,d) for |c|=lc, |d|=ld from (ab|f0)
+    kseen = {}
+    korder = []
+
+    def ket(c, d):
+        key = (c, d)
+        if key in kseen:
+            return key
+        if sum(d) == 0:
+            kseen[key] = ("base",)
+        else:
+            i = next(i for i in range(3) if d[i] > 0)
+            d1 = sub(d, i)
+            kseen[key] = ("rec", i, ket(add(c, i), d1), ket(c, d1))
+        korder.append(key)
+        return key
+
+    finals_cd = [(c, d) for c in cart(lc) for d in cart(ld)]
+    for c, d in finals_cd:
+        ket(c, d)
+
+    def kvar(ab, key):
+        c, d = key
+        if sum(d) == 0:
+            return bvar(ab, c)
+        return f"hk{name3(ab[0])}{name3(ab[1])}_{name3(c)}_{name3(d)}"
+
+    out_map = {}
+    nB, nC, nD = len(cart(lb)), len(cart(lc)), len(cart(ld))
+    for ia, a in enumerate(cart(la)):
+        for ib, b in enumerate(cart(lb)):
+            ab = (a, b)
+            for key in korder:
+                r = kseen[key]
+                if r[0] == "base":
+                    continue
+                em.emit(f"    const double {kvar(ab, key)} = {kvar(ab, r[2])} + CD{xyz[r[1]]} * {kvar(ab, r[3])};")
+                em.flops += 2
+            for ic, c in enumerate(cart(lc)):
+                for idd, d in enumerate(cart(ld)):
+                    idx = ((ia * nB + ib) * nC + ic) * nD + idd
+                    em.emit(f"    out[{idx}] = {kvar(ab, (c, d))};")
+    return out_map
+
+
+def gen_class(la, lb, lc, ld):
+    L = la + lb + lc + ld
+    em = Emitter()
+    name = f"eri_{la}{lb}{lc}{ld}"
+    em.emit(f"// class ({'spd'[la]}{'spd'[lb]}|{'spd'[lc]}{'spd'[ld]}), L = {L}")
+    em.emit(f"__device__ __forceinline__ void {name}(const PrimPair* __restrict__ bp, int nbp,")
+    em.emit(f"        const PrimPair* __restrict__ kp, int nkp, double ABx, double ABy, double ABz,")
+    em.emit(f"        double CDx, double CDy, double CDz, const double* __restrict__ btab, double* __restrict__ out) {{")
+    body = Emitter()
+    accmap, nvrr, targets = gen_vrr(la, lb, lc, ld, body)
+    acc = {t: f"acc{n}" for n, t in enumerate(targets)}
+    em.emit("    " + " ".join(f"double {acc[t]} = 0.0;" for t in targets))
+    em.emit("    for (int ib = 0; ib < nbp; ++ib) {")
+    em.emit("      const PrimPair B = bp[ib];")
+    if la + lb > 0:
+        em.emit("      const double PAx = B.PAx, PAy = B.PAy, PAz = B.PAz, oo2p = B.oo2p;")
+    em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
+    em.emit("        const PrimPair K = kp[ik];")
+    em.emit("        const double s = B.p + K.p, rs = 1.0 / s;")
+    em.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")
+    em.emit("        const double rho = B.p * K.p * rs;")
+    em.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    em.emit("        const double pref = B.K * K.K * sqrt(rs);")
+    em.emit(f"        double Fb[{L + 1}];")
+    em.emit(f"        boys_eval<{L}>(T, btab, Fb);")
+    for m in range(L + 1):
+        em.emit(f"        const double F{m} = pref * Fb[{m}];")
+    if la + lb > 0:
+        em.emit("        const double WPx = -K.p * rs * PQx, WPy = -K.p * rs * PQy, WPz = -K.p * rs * PQz;")
+        em.emit("        const double rop = K.p * rs;")
+    if lc + ld > 0:
+        em.emit("        const double QCx = K.PAx, QCy = K.PAy, QCz = K.PAz, oo2q = K.oo2p;")
+        em.emit("        const double WQx = B.p * rs * PQx, WQy = B.p * rs * PQy, WQz = B.p * rs * PQz;")
+        em.emit("        const double roq = B.p * rs;")
+        if la + lb > 0:
+            em.emit("        const double oo2s = 0.5 * rs;")
+    for ln in body.lines:
+        em.emit("    " + ln)
+    for t in targets:
+        em.emit(f"        {acc[t]} += {accmap[t]};")
+    em.emit("      }")
+    em.emit("    }")
+    hr = Emitter()
+    gen_hrr(la, lb, lc, ld, acc, hr)
+    em.lines += hr.lines
+    em.emit("}")
+    # per primitive quartet: setup ~ 25 flops + boys (approx 2*8 + 3*L + exp~20) + VRR + accumulate
+    per_prim = body.flops + len(targets) + 25 + 16 + (3 * L + 20 if L > 0 else 0)
+    info = dict(cls=f"{la}{lb}{lc}{ld}", L=L, vrr_terms=nvrr, targets=len(targets),
+                flops_per_primitive_quartet=per_prim, flops_hrr=hr.flops)
+    return "\n".join(em.lines), info
+
+
+def main(out_path: str) -> Dict:
+    chunks = ["// Generated by gen_eri.py -- do not edit.", "",
+              "template <int LA, int LB, int LC, int LD>",
+              "__device__ void eri_block(const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp,",
+              "    int nkp, double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,",
+              "    const double* __restrict__ btab, double* __restrict__ out);", ""]
+    infos = []
+    for (la, lb, lc, ld) in quartet_classes():
+        code, info = gen_class(la, lb, lc, ld)
+        chunks.append(code)
+        chunks.append(f"template <> __device__ __forceinline__ void eri_block<{la}, {lb}, {lc}, {ld}>(")
+        chunks.append("    const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp, int nkp,")
+        chunks.append("    double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
+        chunks.append("    const double* __restrict__ btab, double* __restrict__ out) {")
+        chunks.append(f"  eri_{la}{lb}{lc}{ld}(bp, nbp, kp, nkp, ABx, ABy, ABz, CDx, CDy, CDz, btab, out);")
+        chunks.append("}")
+        chunks.append("")
+        infos.append(info)
+    text = "\n".join(chunks) + "\n"
+    if not os.path.exists(out_path) or open(out_path).read() != text:
+        with open(out_path, "w") as fh:
+            fh.write(text)
+    return {i["cls"]: i for i in infos}
+
+
+if __name__ == "__main__":
+    here = os.path.dirname(os.path.abspath(__file__))
+    infos = main(os.path.join(here, "eri_gen.inc"))
+    json.dump(infos, sys.stdout, indent=1)

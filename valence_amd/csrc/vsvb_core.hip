@@ -1,0 +1,1169 @@
+// VSVB energy path on MI355X: context, device preparation kernels, one-electron
+// integrals, orbital-space algebra, energy and adjoint gradient, C ABI.
+//
+// Per call (all on the context's stream, inputs already resident):
+//   geometry  -> primitive-pair data, S_ao / h_ao (Obara-Saika), Schwarz bounds
+//   weights   -> effective orbital weights C (NDF expansion, angn scaling) and the
+//                reference's per-(orbital, shell) weight screen for the ERIs
+//   S = C S_ao C^T, h = C h_ao C^T, P = S^-1 per spin block (closed form of the
+//   first- and second-order cofactors of the spin-orbital overlap), R = C' P C'^T
+//   E1 = sum_s w_s tr(h_s P_s), E2 from the shell-quartet kernels, E = E1 + E2 + E_nuc
+//   gradient: J/K from the same quartets (recomputed, never stored), then
+//   dE/dC_s = 2 P_s C_s F_s (1 - R_s S_ao) with F_s = dE/dR_s.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/vsvb_hip.h"
+#include "vsvb_eri_kernels.h"
+
+using namespace vsvb;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      return fail(VSVB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));                 \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// host helpers: normalization (norm_prim, valence.F90:2264-2306), angn
+// (setangn, valence.F90:2395-2421), Boys table
+// ---------------------------------------------------------------------------
+static double dblfac(int n) {
+  double d = 1.0;
+  for (int i = 3; i <= n; i += 2) d *= (double)i;
+  return d;
+}
+
+static void norm_prim(int l, int n, const double* e, const double* u, double* out) {
+  const double pi32 = std::pow(std::acos(-1.0), 1.5);
+  const double fac = pi32 * dblfac(2 * l - 1) * std::pow(2.0, -l);
+  const double fax = -1.5 - l;
+  for (int i = 0; i < n; ++i) out[i] = u[i] * std::pow(fac * std::pow(2.0 * e[i], fax), -0.5);
+  double s = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s += fac * out[i] * out[j] * std::pow(e[i] + e[j], fax);
+  s = std::pow(s, -0.5);
+  for (int i = 0; i < n; ++i) out[i] *= s;
+}
+
+static double angn_of(int lx, int ly, int lz) {
+  auto ash = [](int l) {
+    double f = 1.0;
+    for (int j = 1; j <= 2 * l - 1; j += 2) f *= (double)j;
+    return std::sqrt(f);
+  };
+  return ash(lx + ly + lz) / (ash(lx) * ash(ly) * ash(lz));
+}
+
+// F_k(T_i), k < kBoysK, by the positive series in long double then downward recursion.
+static std::vector<double> make_boys_table() {
+  std::vector<double> tab((size_t)kBoysN * kBoysK);
+  for (int i = 0; i < kBoysN; ++i) {
+    const long double T = (long double)i * (long double)kBoysDT;
+    const int K = kBoysK - 1;
+    long double term = 1.0L / (2 * K + 1), sum = term;
+    for (int k = 1; k < 2000; ++k) {
+      term *= 2.0L * T / (2 * K + 2 * k + 1);
+      sum += term;
+      if (term < 1e-22L * sum) break;
+    }
+    const long double ex = expl(-T);
+    long double F[kBoysK];
+    F[K] = ex * sum;
+    for (int m = K - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + ex) / (2 * m + 1);
+    for (int m = 0; m < kBoysK; ++m) tab[(size_t)i * kBoysK + m] = (double)F[m];
+  }
+  return tab;
+}
+
+// ---------------------------------------------------------------------------
+// device: geometry preparation
+// ---------------------------------------------------------------------------
+struct PPSrc {
+  int pair, ia, ib, pad;  // pair index, global primitive indices of A and B
+};
+
+__global__ void k_pair_prep(const PPSrc* __restrict__ src, int npp, const Pair* __restrict__ pairs,
+                            const Shell* __restrict__ shells, const double* __restrict__ pexp,
+                            const double* __restrict__ pcoef, const double* __restrict__ xyz, double kfold,
+                            PrimPair* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npp) return;
+  const PPSrc s = src[t];
+  const Pair P = pairs[s.pair];
+  const Shell A = shells[P.A], B = shells[P.B];
+  const double Ax = xyz[3 * A.atom], Ay = xyz[3 * A.atom + 1], Az = xyz[3 * A.atom + 2];
+  const double Bx = xyz[3 * B.atom], By = xyz[3 * B.atom + 1], Bz = xyz[3 * B.atom + 2];
+  const double a = pexp[s.ia], b = pexp[s.ib], p = a + b, rp = 1.0 / p;
+  const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
+  PrimPair o;
+  o.p = p;
+  o.oo2p = 0.5 * rp;
+  o.Px = (a * Ax + b * Bx) * rp;
+  o.Py = (a * Ay + b * By) * rp;
+  o.Pz = (a * Az + b * Bz) * rp;
+  o.PAx = o.Px - Ax;
+  o.PAy = o.Py - Ay;
+  o.PAz = o.Pz - Az;
+  o.K = pcoef[s.ia] * pcoef[s.ib] * exp(-a * b * rp * (ABx * ABx + ABy * ABy + ABz * ABz)) * rp * kfold;
+  o.pad = 0.0;
+  out[t] = o;
+}
+
+__global__ void k_pair_geom(Pair* __restrict__ pairs, int np, const Shell* __restrict__ shells,
+                            const double* __restrict__ xyz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const Shell A = shells[pairs[t].A], B = shells[pairs[t].B];
+  pairs[t].ABx = xyz[3 * A.atom] - xyz[3 * B.atom];
+  pairs[t].ABy = xyz[3 * A.atom + 1] - xyz[3 * B.atom + 1];
+  pairs[t].ABz = xyz[3 * A.atom + 2] - xyz[3 * B.atom + 2];
+}
+
+// ---------------------------------------------------------------------------
+// device: one-electron integrals (Obara-Saika), one thread per shell pair
+// ---------------------------------------------------------------------------
+// cartesian tuples of total order 0..4 in CCA order, cumulative index
+__device__ __forceinline__ constexpr int cart_off(int l) { return l * (l + 1) * (l + 2) / 6; }
+__device__ __forceinline__ constexpr int cart_idx(int lx, int ly, int lz) {
+  return cart_off(lx + ly + lz) + ((ly + lz) * (ly + lz + 1)) / 2 + lz;
+}
+__device__ __forceinline__ void cart_of(int l, int c, int& lx, int& ly, int& lz) {
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= c) ++i;
+  const int j = c - i * (i + 1) / 2;
+  lx = l - i;
+  ly = i - j;
+  lz = j;
+}
+
+template <int LA, int LB>
+__global__ void __launch_bounds__(64) k_one_e(const Pair* __restrict__ pairs, int p0, int np,
+                                              const Shell* __restrict__ shells, const double* __restrict__ pexp,
+                                              const double* __restrict__ pcoef, const double* __restrict__ xyz,
+                                              const double* __restrict__ charge, int natom,
+                                              const double* __restrict__ boys, double* __restrict__ S,
+                                              double* __restrict__ H, int nbf) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
+  constexpr int NE = cart_off(L + 1);        // cartesians of order 0..L
+  constexpr int NBc = cart_off(LB + 1);      // cartesians of order 0..LB
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const Pair PR = pairs[p0 + t];
+  const Shell A = shells[PR.A], B = shells[PR.B];
+  const double Ax = xyz[3 * A.atom], Ay = xyz[3 * A.atom + 1], Az = xyz[3 * A.atom + 2];
+  const double Bx = xyz[3 * B.atom], By = xyz[3 * B.atom + 1], Bz = xyz[3 * B.atom + 2];
+  const double AB[3] = {Ax - Bx, Ay - By, Az - Bz};
+  double s[NA * NB], tk[NA * NB], v[NA * NB];
+#pragma unroll
+  for (int k = 0; k < NA * NB; ++k) s[k] = tk[k] = v[k] = 0.0;
+  for (int ia = 0; ia < A.nprim; ++ia)
+    for (int ib = 0; ib < B.nprim; ++ib) {
+      const double a = pexp[A.prim0 + ia], b = pexp[B.prim0 + ib], p = a + b, rp = 1.0 / p;
+      const double mu = a * b * rp;
+      const double cc = pcoef[A.prim0 + ia] * pcoef[B.prim0 + ib];
+      const double Pc[3] = {(a * Ax + b * Bx) * rp, (a * Ay + b * By) * rp, (a * Az + b * Bz) * rp};
+      const double PA[3] = {Pc[0] - Ax, Pc[1] - Ay, Pc[2] - Az};
+      const double PB[3] = {Pc[0] - Bx, Pc[1] - By, Pc[2] - Bz};
+      // 1D overlaps, i <= LA, j <= LB + 2
+      double S1[3][LA + 1][LB + 3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        S1[d][0][0] = sqrt(M_PI * rp) * exp(-mu * AB[d] * AB[d]);
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB + 2; ++j) {
+            if (i == 0 && j == 0) continue;
+            double val;
+            if (i > 0) {
+              val = PA[d] * S1[d][i - 1][j];
+              if (i > 1) val += (i - 1) * 0.5 * rp * S1[d][i - 2][j];
+              if (j > 0) val += j * 0.5 * rp * S1[d][i - 1][j - 1];
+            } else {
+              val = PB[d] * S1[d][i][j - 1];
+              if (j > 1) val += (j - 1) * 0.5 * rp * S1[d][i][j - 2];
+            }
+            S1[d][i][j] = val;
+          }
+      }
+#pragma unroll
+      for (int ca = 0; ca < NA; ++ca)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
+          int ax, ay, az, bx, by, bz;
+          cart_of(LA, ca, ax, ay, az);
+          cart_of(LB, cb, bx, by, bz);
+          const int ai[3] = {ax, ay, az}, bj[3] = {bx, by, bz};
+          double sv[3], tv[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int i = ai[d], j = bj[d];
+            sv[d] = S1[d][i][j];
+            const double km = j >= 2 ? S1[d][i][j - 2] : 0.0;
+            tv[d] = -0.5 * (j * (j - 1) * km - 2.0 * b * (2 * j + 1) * sv[d] + 4.0 * b * b * S1[d][i][j + 2]);
+          }
+          s[ca * NB + cb] += cc * sv[0] * sv[1] * sv[2];
+          tk[ca * NB + cb] += cc * (tv[0] * sv[1] * sv[2] + sv[0] * tv[1] * sv[2] + sv[0] * sv[1] * tv[2]);
+        }
+      // nuclear attraction: VRR [e|0]^(m), |e| <= L, then HRR to [a|b]
+      const double pre = 2.0 * M_PI * rp * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2])) * cc;
+      for (int c = 0; c < natom; ++c) {
+        const double Z = charge[c];
+        if (fabs(Z) <= 1e-12) continue;  // int1e skips |Z| <= 1e-12 (valence.F90:3146)
+        const double PC[3] = {Pc[0] - xyz[3 * c], Pc[1] - xyz[3 * c + 1], Pc[2] - xyz[3 * c + 2]};
+        double F[L + 1];
+        boys_eval<L>(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), boys, F);
+        double Vm[L + 1][NE];
+#pragma unroll
+        for (int m = 0; m <= L; ++m) Vm[m][0] = -Z * pre * F[m];
+#pragma unroll
+        for (int l = 1; l <= L; ++l)
+#pragma unroll
+          for (int c2 = 0; c2 < ncart(l); ++c2) {
+            int ex, ey, ez;
+            cart_of(l, c2, ex, ey, ez);
+            const int e[3] = {ex, ey, ez};
+            const int i = ex > 0 ? 0 : (ey > 0 ? 1 : 2);
+            int e1[3] = {ex, ey, ez};
+            e1[i] -= 1;
+            const int i1 = cart_idx(e1[0], e1[1], e1[2]);
+            const int nn = e1[i];
+            int i2 = 0;
+            if (nn > 0) {
+              int e2[3] = {e1[0], e1[1], e1[2]};
+              e2[i] -= 1;
+              i2 = cart_idx(e2[0], e2[1], e2[2]);
+            }
+            const int ie = cart_off(l) + c2;
+#pragma unroll
+            for (int m = 0; m <= L - l; ++m) {
+              double val = PA[i] * Vm[m][i1] - PC[i] * Vm[m + 1][i1];
+              if (nn > 0) val += nn * 0.5 * rp * (Vm[m][i2] - Vm[m + 1][i2]);
+              Vm[m][ie] = val;
+            }
+            (void)e;
+          }
+        // HRR on b: W[e][b] = W[e+1_i][b-1_i] + AB_i W[e][b-1_i]
+        double W[NE][NBc];
+#pragma unroll
+        for (int ie = 0; ie < NE; ++ie) W[ie][0] = Vm[0][ie];
+#pragma unroll
+        for (int lb = 1; lb <= LB; ++lb)
+#pragma unroll
+          for (int cbb = 0; cbb < ncart(lb); ++cbb) {
+            int bx, by, bz;
+            cart_of(lb, cbb, bx, by, bz);
+            const int i = bx > 0 ? 0 : (by > 0 ? 1 : 2);
+            int b1[3] = {bx, by, bz};
+            b1[i] -= 1;
+            const int ib1 = cart_idx(b1[0], b1[1], b1[2]);
+            const int ibb = cart_off(lb) + cbb;
+#pragma unroll
+            for (int le = 0; le <= L - lb; ++le)
+#pragma unroll
+              for (int ce = 0; ce < ncart(le); ++ce) {
+                int ex, ey, ez;
+                cart_of(le, ce, ex, ey, ez);
+                int ep[3] = {ex, ey, ez};
+                ep[i] += 1;
+                const int iep = cart_idx(ep[0], ep[1], ep[2]);
+                const int iee = cart_off(le) + ce;
+                W[iee][ibb] = W[iep][ib1] + AB[i] * W[iee][ib1];
+              }
+          }
+#pragma unroll
+        for (int ca = 0; ca < NA; ++ca)
+#pragma unroll
+          for (int cb = 0; cb < NB; ++cb) v[ca * NB + cb] += W[cart_off(LA) + ca][cart_off(LB) + cb];
+      }
+    }
+#pragma unroll
+  for (int ca = 0; ca < NA; ++ca)
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const int mu = A.ao + ca, nu = B.ao + cb;
+      const double sv = s[ca * NB + cb], hv = tk[ca * NB + cb] + v[ca * NB + cb];
+      S[(size_t)mu * nbf + nu] = sv;
+      S[(size_t)nu * nbf + mu] = sv;
+      H[(size_t)mu * nbf + nu] = hv;
+      H[(size_t)nu * nbf + mu] = hv;
+    }
+}
+
+static hipError_t launch_one_e(int pc, const Pair* pairs, int p0, int np, const Shell* shells, const double* pexp,
+                               const double* pcoef, const double* xyz, const double* charge, int natom,
+                               const double* boys, double* S, double* H, int nbf, hipStream_t st) {
+  if (np <= 0) return hipSuccess;
+  const dim3 g((np + 63) / 64), b(64);
+#define LAUNCH(LA, LB) \
+  hipLaunchKernelGGL((k_one_e<LA, LB>), g, b, 0, st, pairs, p0, np, shells, pexp, pcoef, xyz, charge, natom, boys, S, H, nbf)
+  switch (pc) {
+    case 0: LAUNCH(0, 0); break;
+    case 1: LAUNCH(1, 0); break;
+    case 2: LAUNCH(1, 1); break;
+    case 3: LAUNCH(2, 0); break;
+    case 4: LAUNCH(2, 1); break;
+    case 5: LAUNCH(2, 2); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef LAUNCH
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// device: orbital weights
+// ---------------------------------------------------------------------------
+struct OrbEntry {
+  int orb;   // owning orbital
+  int kind;  // 0: AO, 1: NDF reference
+  int tgt;   // AO index (kind 0) or NDF orbital index (kind 1)
+  int pad;
+};
+
+// NDF self-overlap scale n_f^-1/2 (normal over the NDF range, valence.F90:144).
+__global__ void k_ndf_scale(const OrbEntry* __restrict__ ent, const int* __restrict__ orb_start,
+                            const double* __restrict__ coeff, const double* __restrict__ angn,
+                            const double* __restrict__ S, int nbf, int first_ndf, int ndf, double* __restrict__ scale) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ndf) return;
+  const int o = first_ndf + f;
+  double s = 0.0;
+  for (int k = orb_start[o]; k < orb_start[o + 1]; ++k)
+    for (int l = orb_start[o]; l < orb_start[o + 1]; ++l)
+      s += angn[ent[k].tgt] * coeff[k] * angn[ent[l].tgt] * coeff[l] * S[(size_t)ent[k].tgt * nbf + ent[l].tgt];
+  scale[f] = 1.0 / sqrt(s);
+}
+
+// Effective weights Ct[p][mu] (angn folded) and the ERI weights Cs with the
+// reference's per-(orbital, shell) screen: shell kept iff sum of its squared
+// normalized raw weights > dtol (int2e, valence.F90:3296-3300).
+__global__ void k_build_orbitals(const OrbEntry* __restrict__ ent, const int* __restrict__ orb_start,
+                                 const double* __restrict__ coeff, const double* __restrict__ angn,
+                                 const double* __restrict__ ndf_scale, int first_ndf, const double* __restrict__ S,
+                                 const int* __restrict__ ao_shell, const Shell* __restrict__ shells, int nbf,
+                                 int nsp, double dtol, double* __restrict__ Ct, double* __restrict__ Cs,
+                                 int* __restrict__ nmasked) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nsp) return;
+  double* c = Ct + (size_t)p * nbf;
+  double* cs = Cs + (size_t)p * nbf;
+  for (int mu = 0; mu < nbf; ++mu) c[mu] = 0.0;
+  for (int k = orb_start[p]; k < orb_start[p + 1]; ++k) {
+    const OrbEntry e = ent[k];
+    if (e.kind == 0) {
+      c[e.tgt] += angn[e.tgt] * coeff[k];
+    } else {
+      const int f = e.tgt;
+      const double w = coeff[k] * ndf_scale[f - first_ndf];
+      for (int l = orb_start[f]; l < orb_start[f + 1]; ++l) c[ent[l].tgt] += angn[ent[l].tgt] * coeff[l] * w;
+    }
+  }
+  // self overlap over the nonzero support
+  double spp = 0.0;
+  for (int mu = 0; mu < nbf; ++mu) {
+    if (c[mu] == 0.0) continue;
+    const double* row = S + (size_t)mu * nbf;
+    for (int nu = 0; nu < nbf; ++nu)
+      if (c[nu] != 0.0) spp += c[mu] * c[nu] * row[nu];
+  }
+  const double inv = 1.0 / spp;
+  int masked = 0;
+  int mu = 0;
+  while (mu < nbf) {
+    const Shell sh = shells[ao_shell[mu]];
+    const int n = ncart(sh.l);
+    double sum = 0.0;
+    bool nz = false;
+    for (int k = 0; k < n; ++k) {
+      const double raw = c[mu + k] / angn[mu + k];
+      sum += raw * raw * inv;
+      nz |= c[mu + k] != 0.0;
+    }
+    const bool keep = sum > dtol;
+    if (nz && !keep) ++masked;
+    for (int k = 0; k < n; ++k) cs[mu + k] = keep ? c[mu + k] : 0.0;
+    mu += n;
+  }
+  if (masked) atomicAdd(nmasked, masked);
+}
+
+// ---------------------------------------------------------------------------
+// device: small dense algebra (row-major)
+// ---------------------------------------------------------------------------
+// C[M x N] = alpha * op(A) * op(B) + beta * C;  op(A) is M x K.
+__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+                                              int ta, const double* __restrict__ B, int ldb, int tb, double beta,
+                                              double* __restrict__ C, int ldc) {
+  __shared__ double As[16][17], Bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    {
+      const int r = blockIdx.y * 16 + ty, k = k0 + tx;
+      double v = 0.0;
+      if (r < M && k < K) v = ta ? A[(size_t)k * lda + r] : A[(size_t)r * lda + k];
+      As[ty][tx] = v;
+    }
+    {
+      const int k = k0 + ty, c = blockIdx.x * 16 + tx;
+      double v = 0.0;
+      if (k < K && c < N) v = tb ? B[(size_t)c * ldb + k] : B[(size_t)k * ldb + c];
+      Bs[ty][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    __syncthreads();
+  }
+  if (row < M && col < N) {
+    double* c = C + (size_t)row * ldc + col;
+    *c = alpha * acc + (beta == 0.0 ? 0.0 : beta * *c);
+  }
+}
+
+static hipError_t gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
+                       const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  dim3 g((N + 15) / 16, (M + 15) / 16);
+  hipLaunchKernelGGL(k_gemm, g, dim3(256), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
+  return hipGetLastError();
+}
+
+// Copy the n x n block starting at (off, off) of an ld-strided matrix.
+__global__ void k_copy_block(const double* __restrict__ src, int ld, int off, int n, double* __restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  dst[t] = src[(size_t)(off + t / n) * ld + off + t % n];
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting needed: every
+// leading block of an SPD matrix has positive diagonal).  One workgroup.
+__global__ void __launch_bounds__(1024) k_spd_inverse(double* __restrict__ a, int n, int* __restrict__ status) {
+  extern __shared__ double colk[];
+  for (int k = 0; k < n; ++k) {
+    const double piv = a[(size_t)k * n + k];
+    __syncthreads();
+    if (!(piv > 0.0) || !isfinite(piv)) {
+      if (threadIdx.x == 0) *status = 1;
+      return;
+    }
+    const double ip = 1.0 / piv;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = a[(size_t)i * n + k];
+    __syncthreads();
+    // scale row k
+    for (int j = threadIdx.x; j < n; j += blockDim.x) a[(size_t)k * n + j] = (j == k ? 1.0 : a[(size_t)k * n + j]) * ip;
+    __syncthreads();
+    const size_t nn = (size_t)n * n;
+    for (size_t t = threadIdx.x; t < nn; t += blockDim.x) {
+      const int i = (int)(t / n), j = (int)(t % n);
+      if (i == k) continue;
+      const double f = colk[i];
+      const double akj = a[(size_t)k * n + j];
+      a[t] = (j == k ? 0.0 : a[t]) - f * akj;
+    }
+    __syncthreads();
+  }
+}
+
+// out[0] += w * sum_ij A[off+i][off+j] * B[i][j]  (A ld-strided block, B n x n)
+__global__ void k_trace_prod(const double* __restrict__ A, int ld, int off, const double* __restrict__ B, int n,
+                             double w, double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) s += A[(size_t)(off + t / n) * ld + off + t % n] * B[t];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] += w * red[0];
+}
+
+// Sum of many partials into out[w] (deterministic order).
+__global__ void k_reduce_partials(const double* __restrict__ partial, long n, int nw, double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int w = 0; w < nw; ++w) {
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) s += partial[i * nw + w];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[w] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// M <- a*(X + X^T) + b*Y (elementwise, n x n); X, Y may alias nothing of M.
+__global__ void k_sym_combine(const double* __restrict__ X, double a, const double* __restrict__ Y, double b, int n,
+                              double* __restrict__ M) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  double v = a * (X[t] + X[(size_t)j * n + i]);
+  if (Y) v += b * Y[t];
+  M[t] = v;
+}
+
+// grad[k] = angn[mu_k] * G[orb_k][mu_k] for AO entries (NDF-carrying inputs are rejected on the host).
+__global__ void k_gather_grad(const OrbEntry* __restrict__ ent, int totlen, int nsp, const double* __restrict__ angn,
+                              const double* __restrict__ G, int nbf, double* __restrict__ grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= totlen) return;
+  const OrbEntry e = ent[k];
+  grad[k] = (e.kind == 0 && e.orb < nsp) ? angn[e.tgt] * G[(size_t)e.orb * nbf + e.tgt] : 0.0;
+}
+
+__global__ void k_final_energy(const double* __restrict__ e1, const double* __restrict__ e2, double enuc,
+                               double* __restrict__ out) {
+  out[0] = e1[0] + e2[0] + enuc;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct vsvb_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double tau = 1e-17;
+  // sizes
+  int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
+  int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;
+  double dtol = 0.0;
+  int pair_off[kNPairClass + 1] = {0};
+  // host copies
+  std::vector<double> coords, charges, coeff;
+  std::vector<OrbEntry> ent;
+  std::vector<int> orb_start;
+  // device
+  DevBuf<Shell> d_shells;
+  DevBuf<Pair> d_pairs;
+  DevBuf<PPSrc> d_ppsrc;
+  DevBuf<PrimPair> d_pp;
+  DevBuf<double> d_pexp, d_pcoef, d_xyz, d_charge, d_boys, d_angn;
+  DevBuf<int> d_ao_shell, d_orb_start, d_status;
+  DevBuf<OrbEntry> d_ent;
+  DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
+  DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
+  DevBuf<unsigned long long> d_counters;
+  size_t partial_per_class[kNQuartetClass] = {0};
+  int blocks_per_class[kNQuartetClass] = {0};
+  int ntiles_class[kNQuartetClass] = {0};
+  size_t partial_total = 0;
+  bool geom_ready = false;
+  double enuc = 0.0;
+  double last_counters[4] = {0, 0, 0, 0};
+};
+
+static double nuclear_repulsion(int natom, const double* xyz, const double* Z) {
+  // get_nuclear_repulsion_energy (src/tools_module.F90:19-40)
+  double e = 0.0;
+  for (int i = 1; i < natom; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double zij = Z[i] * Z[j];
+      if (std::fabs(zij) > 1e-12) {
+        const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        if (rsq > 1e-5) e += zij * std::pow(rsq, -0.5);
+      }
+    }
+  return e;
+}
+
+static const int kPairLs[kNPairClass][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+
+static int pair_class_of(int la, int lb) {
+  for (int c = 0; c < kNPairClass; ++c)
+    if (kPairLs[c][0] == la && kPairLs[c][1] == lb) return c;
+  return -1;
+}
+
+static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st) {
+  switch (bc) {
+    case 0: return launch_quartet_b0(kc, a, jk, nblocks, st);
+    case 1: return launch_quartet_b1(kc, a, jk, nblocks, st);
+    case 2: return launch_quartet_b2(kc, a, jk, nblocks, st);
+    case 3: return launch_quartet_b3(kc, a, jk, nblocks, st);
+    case 4: return launch_quartet_b4(kc, a, jk, nblocks, st);
+    case 5: return launch_quartet_b5(kc, a, jk, nblocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+static hipError_t launch_schwarz(int pc, Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np,
+                                 hipStream_t st) {
+  switch (pc) {
+    case 0: return launch_schwarz_b0(pairs, pp, boys, p0, np, st);
+    case 1: return launch_schwarz_b1(pairs, pp, boys, p0, np, st);
+    case 2: return launch_schwarz_b2(pairs, pp, boys, p0, np, st);
+    case 3: return launch_schwarz_b3(pairs, pp, boys, p0, np, st);
+    case 4: return launch_schwarz_b4(pairs, pp, boys, p0, np, st);
+    case 5: return launch_schwarz_b5(pairs, pp, boys, p0, np, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+static int quartet_class_index(int bc, int kc) { return bc * (bc + 1) / 2 + kc; }
+
+static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_context* c) {
+  if (!s) return fail(VSVB_ERR_ARG, "null system");
+  if (s->npair > 0) return fail(VSVB_ERR_UNSUPPORTED, "spin-coupled pairs are not handled by this context type");
+  if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
+  c->device = opt ? opt->device : 0;
+  c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&c->ev0));
+  HIP_OK(hipEventCreate(&c->ev1));
+  c->natom = s->natom;
+  c->npair = s->npair;
+  c->nunpd = s->nunpd;
+  c->ndocc = s->ndocc;
+  c->ndf = s->ndf;
+  c->norbs = s->norbs;
+  c->totlen = s->totlen;
+  c->nsp = 2 * s->npair + s->nunpd + s->ndocc;
+  c->first_ndf = s->norbs - s->ndf;
+  c->dtol = std::pow(10.0, -(double)s->ntol_d);
+  if (c->nsp < 1) return fail(VSVB_ERR_ARG, "no electrons");
+  // normalized primitive weights per type shell
+  std::vector<double> con((size_t)s->nprim);
+  for (int ts = 0; ts < s->nshell_type; ++ts) {
+    const int p0 = s->shell_prim_start[ts], p1 = s->shell_prim_start[ts + 1];
+    if (s->shell_l[ts] < 0 || s->shell_l[ts] > kLMax) return fail(VSVB_ERR_UNSUPPORTED, "only s, p and d shells");
+    norm_prim(s->shell_l[ts], p1 - p0, s->prim_exp + p0, s->prim_coef + p0, con.data() + p0);
+  }
+  // global shells, per-shell primitive copies, AO map
+  std::vector<Shell> shells;
+  std::vector<double> pexp, pcoef, angn;
+  std::vector<int> ao_shell, atom_sh0(s->natom + 1);
+  int ao = 0;
+  for (int a = 0; a < s->natom; ++a) {
+    const int t = s->atom_type[a];
+    if (t < 0 || t >= s->ntype) return fail(VSVB_ERR_ARG, "atom type out of range");
+    atom_sh0[a] = (int)shells.size();
+    for (int ts = s->type_shell_start[t]; ts < s->type_shell_start[t + 1]; ++ts) {
+      Shell sh;
+      sh.atom = a;
+      sh.l = s->shell_l[ts];
+      sh.nprim = s->shell_prim_start[ts + 1] - s->shell_prim_start[ts];
+      sh.prim0 = (int)pexp.size();
+      sh.ao = ao;
+      for (int k = 0; k < sh.nprim; ++k) {
+        pexp.push_back(s->prim_exp[s->shell_prim_start[ts] + k]);
+        pcoef.push_back(con[s->shell_prim_start[ts] + k]);
+      }
+      for (int i = 0; i <= sh.l; ++i)
+        for (int j = 0; j <= i; ++j) {
+          angn.push_back(angn_of(sh.l - i, i - j, j));
+          ao_shell.push_back((int)shells.size());
+        }
+      ao += ncart(sh.l);
+      shells.push_back(sh);
+    }
+  }
+  atom_sh0[s->natom] = (int)shells.size();
+  c->nsh = (int)shells.size();
+  c->nbf = ao;
+  // shell pairs: class-major, by decreasing primitive-pair count
+  struct HP { Pair p; int cls; };
+  std::vector<HP> hp;
+  for (int M = 0; M < c->nsh; ++M)
+    for (int N = 0; N <= M; ++N) {
+      HP h{};
+      int A = M, B = N;
+      if (shells[A].l < shells[B].l) std::swap(A, B);
+      h.p.A = A;
+      h.p.B = B;
+      h.p.npp = shells[A].nprim * shells[B].nprim;
+      h.p.offdiag = M != N;
+      h.cls = pair_class_of(shells[A].l, shells[B].l);
+      hp.push_back(h);
+    }
+  std::stable_sort(hp.begin(), hp.end(), [](const HP& x, const HP& y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    return x.p.npp > y.p.npp;
+  });
+  std::vector<Pair> pairs(hp.size());
+  std::vector<PPSrc> ppsrc;
+  for (size_t i = 0; i < hp.size(); ++i) {
+    pairs[i] = hp[i].p;
+    pairs[i].pp0 = (int)ppsrc.size();
+    const Shell& A = shells[pairs[i].A];
+    const Shell& B = shells[pairs[i].B];
+    for (int ia = 0; ia < A.nprim; ++ia)
+      for (int ib = 0; ib < B.nprim; ++ib) ppsrc.push_back(PPSrc{(int)i, A.prim0 + ia, B.prim0 + ib, 0});
+  }
+  for (int cl = 0; cl <= kNPairClass; ++cl) c->pair_off[cl] = 0;
+  for (auto& h : hp) c->pair_off[h.cls + 1]++;
+  for (int cl = 0; cl < kNPairClass; ++cl) c->pair_off[cl + 1] += c->pair_off[cl];
+  c->npairs = (int)pairs.size();
+  c->npp = (int)ppsrc.size();
+  // orbital entries: map each weight to an AO of the molecule (OBS-local numbering, xm_input)
+  c->ent.resize(std::max(1, s->totlen));
+  c->orb_start.assign(s->orb_start, s->orb_start + s->norbs + 1);
+  for (int p = 0; p < s->norbs; ++p) {
+    const int na = s->orb_natom[p];
+    if (na < 1 || na > s->mxctr) return fail(VSVB_ERR_ARG, "orbital OBS atom count out of range");
+    for (int k = s->orb_start[p]; k < s->orb_start[p + 1]; ++k) {
+      OrbEntry e{p, 0, -1, 0};
+      const int xp = s->xpset[k];
+      if (xp < 1) {
+        const int f = xp + s->norbs - 1;
+        if (f < c->first_ndf || f >= s->norbs) return fail(VSVB_ERR_ARG, "NDF reference out of range");
+        if (p >= c->first_ndf) return fail(VSVB_ERR_UNSUPPORTED, "an NDF expanded in other NDFs");
+        e.kind = 1;
+        e.tgt = f;
+      } else {
+        int kk = xp - 1;
+        for (int ia = 0; ia < na; ++ia) {
+          const int atom = s->orb_atoms[p * s->mxctr + ia];
+          if (atom < 0 || atom >= s->natom) return fail(VSVB_ERR_ARG, "OBS atom out of range");
+          const int a0 = shells[atom_sh0[atom]].ao;
+          const int a1 = atom_sh0[atom + 1] < c->nsh ? shells[atom_sh0[atom + 1]].ao : c->nbf;
+          if (kk < a1 - a0) {
+            e.tgt = a0 + kk;
+            break;
+          }
+          kk -= a1 - a0;
+        }
+        if (e.tgt < 0) return fail(VSVB_ERR_ARG, "AO index beyond the orbital's OBS");
+      }
+      c->ent[k] = e;
+    }
+  }
+  // host copies of inputs
+  c->coords.assign(s->coords, s->coords + 3 * s->natom);
+  c->charges.resize(s->natom);
+  for (int a = 0; a < s->natom; ++a) c->charges[a] = s->nuc_charge[s->atom_type[a]];
+  c->coeff.assign(s->coeff, s->coeff + s->totlen);
+  // device allocations (all of them, here)
+  const size_t nbf2 = (size_t)c->nbf * c->nbf;
+  HIP_OK(c->d_shells.alloc(c->nsh));
+  HIP_OK(c->d_pairs.alloc(c->npairs));
+  HIP_OK(c->d_ppsrc.alloc(c->npp));
+  HIP_OK(c->d_pp.alloc(c->npp));
+  HIP_OK(c->d_pexp.alloc(pexp.size()));
+  HIP_OK(c->d_pcoef.alloc(pcoef.size()));
+  HIP_OK(c->d_xyz.alloc(3 * c->natom));
+  HIP_OK(c->d_charge.alloc(c->natom));
+  HIP_OK(c->d_boys.alloc((size_t)kBoysN * kBoysK));
+  HIP_OK(c->d_angn.alloc(c->nbf));
+  HIP_OK(c->d_ao_shell.alloc(c->nbf));
+  HIP_OK(c->d_orb_start.alloc(s->norbs + 1));
+  HIP_OK(c->d_status.alloc(4));
+  HIP_OK(c->d_ent.alloc(c->ent.size()));
+  HIP_OK(c->d_coeff.alloc(std::max(1, s->totlen)));
+  HIP_OK(c->d_ndf_scale.alloc(std::max(1, s->ndf)));
+  HIP_OK(c->d_S.alloc(nbf2));
+  HIP_OK(c->d_H.alloc(nbf2));
+  HIP_OK(c->d_Ct.alloc((size_t)c->nsp * c->nbf));
+  HIP_OK(c->d_Cs.alloc((size_t)c->nsp * c->nbf));
+  HIP_OK(c->d_Sorb.alloc((size_t)c->nsp * c->nsp));
+  HIP_OK(c->d_Horb.alloc((size_t)c->nsp * c->nsp));
+  HIP_OK(c->d_P.alloc(2 * (size_t)c->nsp * c->nsp));
+  HIP_OK(c->d_R.alloc(2 * nbf2));
+  HIP_OK(c->d_tmp.alloc(std::max(nbf2, (size_t)c->nsp * c->nbf)));
+  HIP_OK(c->d_tmp2.alloc(std::max(nbf2, (size_t)c->nsp * c->nbf)));
+  HIP_OK(c->d_J.alloc(nbf2));
+  HIP_OK(c->d_K.alloc(2 * nbf2));
+  HIP_OK(c->d_F.alloc(2 * nbf2));
+  HIP_OK(c->d_gradC.alloc((size_t)c->nsp * c->nbf));
+  HIP_OK(c->d_Jsym.alloc(nbf2));
+  HIP_OK(c->d_Rfull.alloc(nbf2));
+  HIP_OK(c->d_X.alloc((size_t)c->nsp * c->nbf));
+  HIP_OK(c->d_Z.alloc((size_t)c->nsp * c->nbf));
+  HIP_OK(c->d_grad.alloc(std::max(1, s->totlen)));
+  HIP_OK(c->d_scal.alloc(16));
+  HIP_OK(c->d_counters.alloc(4));
+  // quartet launch geometry and partial buffer
+  c->partial_total = 0;
+  for (int bc = 0; bc < kNPairClass; ++bc)
+    for (int kc = 0; kc <= bc; ++kc) {
+      const int q = quartet_class_index(bc, kc);
+      const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+      const int tb = (nb + kTile - 1) / kTile, tk = (nk + kTile - 1) / kTile;
+      const int nt = (bc == kc) ? tb * (tb + 1) / 2 : tb * tk;
+      c->ntiles_class[q] = (nb > 0 && nk > 0) ? nt : 0;
+      c->blocks_per_class[q] = std::min(c->ntiles_class[q], 2048);
+      c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;
+      c->partial_total += c->partial_per_class[q];
+    }
+  HIP_OK(c->d_partial.alloc(std::max<size_t>(c->partial_total, 1) * 2));
+  // uploads
+  auto boys = make_boys_table();
+  HIP_OK(hipMemcpy(c->d_shells.p, shells.data(), shells.size() * sizeof(Shell), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_pairs.p, pairs.data(), pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_ppsrc.p, ppsrc.data(), ppsrc.size() * sizeof(PPSrc), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_pexp.p, pexp.data(), pexp.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_pcoef.p, pcoef.data(), pcoef.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_charge.p, c->charges.data(), c->natom * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_boys.p, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_angn.p, angn.data(), angn.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_ao_shell.p, ao_shell.data(), ao_shell.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_orb_start.p, c->orb_start.data(), c->orb_start.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_ent.p, c->ent.data(), c->ent.size() * sizeof(OrbEntry), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_coeff.p, c->coeff.data(), c->totlen * sizeof(double), hipMemcpyHostToDevice));
+  return VSVB_OK;
+}
+
+// geometry-dependent device state
+static int prepare_geometry(vsvb_context* c, const double* coords) {
+  if (coords) c->coords.assign(coords, coords + 3 * c->natom);
+  if (c->geom_ready && !coords) return VSVB_OK;
+  hipStream_t st = c->stream;
+  c->enuc = nuclear_repulsion(c->natom, c->coords.data(), c->charges.data());
+  HIP_OK(hipMemcpyAsync(c->d_xyz.p, c->coords.data(), 3 * c->natom * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pair_geom, dim3((c->npairs + 255) / 256), dim3(256), 0, st, c->d_pairs.p, c->npairs,
+                     c->d_shells.p, c->d_xyz.p);
+  HIP_OK(hipGetLastError());
+  const double kfold = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+  hipLaunchKernelGGL(k_pair_prep, dim3((c->npp + 255) / 256), dim3(256), 0, st, c->d_ppsrc.p, c->npp, c->d_pairs.p,
+                     c->d_shells.p, c->d_pexp.p, c->d_pcoef.p, c->d_xyz.p, kfold, c->d_pp.p);
+  HIP_OK(hipGetLastError());
+  for (int pc = 0; pc < kNPairClass; ++pc) {
+    const int p0 = c->pair_off[pc], np = c->pair_off[pc + 1] - p0;
+    HIP_OK(launch_one_e(pc, c->d_pairs.p, p0, np, c->d_shells.p, c->d_pexp.p, c->d_pcoef.p, c->d_xyz.p,
+                        c->d_charge.p, c->natom, c->d_boys.p, c->d_S.p, c->d_H.p, c->nbf, st));
+    HIP_OK(launch_schwarz(pc, c->d_pairs.p, c->d_pp.p, c->d_boys.p, p0, np, st));
+  }
+  c->geom_ready = true;
+  return VSVB_OK;
+}
+
+// weights -> Ct, Cs, orbital S and h, P per spin block, R per spin block, E1.
+// Spin blocks (npair = 0): alpha = all spatial orbitals (unpaired then docc),
+// beta = the trailing ndocc (docc); closed shell when nunpd = 0.
+static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const double* coeff_dev) {
+  hipStream_t st = c->stream;
+  if (coeff_host) {
+    c->coeff.assign(coeff_host, coeff_host + c->totlen);
+    HIP_OK(hipMemcpyAsync(c->d_coeff.p, c->coeff.data(), c->totlen * sizeof(double), hipMemcpyHostToDevice, st));
+  } else if (coeff_dev) {
+    HIP_OK(hipMemcpyAsync(c->d_coeff.p, coeff_dev, c->totlen * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  const int n = c->nbf, ns = c->nsp;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->d_scal.p, (size_t)16, 0.0);
+  HIP_OK(hipMemsetAsync(c->d_status.p, 0, 4 * sizeof(int), st));
+  if (c->ndf > 0) {
+    hipLaunchKernelGGL(k_ndf_scale, dim3((c->ndf + 63) / 64), dim3(64), 0, st, c->d_ent.p, c->d_orb_start.p,
+                       c->d_coeff.p, c->d_angn.p, c->d_S.p, n, c->first_ndf, c->ndf, c->d_ndf_scale.p);
+    HIP_OK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_build_orbitals, dim3((ns + 63) / 64), dim3(64), 0, st, c->d_ent.p, c->d_orb_start.p,
+                     c->d_coeff.p, c->d_angn.p, c->d_ndf_scale.p, c->first_ndf, c->d_S.p, c->d_ao_shell.p,
+                     c->d_shells.p, n, ns, c->dtol, c->d_Ct.p, c->d_Cs.p, c->d_status.p + 1);
+  HIP_OK(hipGetLastError());
+  // S_orb = Ct S Ct^T, h_orb = Ct H Ct^T
+  HIP_OK(gemm(st, ns, n, n, 1.0, c->d_Ct.p, n, false, c->d_S.p, n, false, 0.0, c->d_tmp.p, n));
+  HIP_OK(gemm(st, ns, ns, n, 1.0, c->d_tmp.p, n, false, c->d_Ct.p, n, true, 0.0, c->d_Sorb.p, ns));
+  HIP_OK(gemm(st, ns, n, n, 1.0, c->d_Ct.p, n, false, c->d_H.p, n, false, 0.0, c->d_tmp.p, n));
+  HIP_OK(gemm(st, ns, ns, n, 1.0, c->d_tmp.p, n, false, c->d_Ct.p, n, true, 0.0, c->d_Horb.p, ns));
+  const int nspin = c->nunpd > 0 ? 2 : 1;
+  for (int s = 0; s < nspin; ++s) {
+    const int off = s == 0 ? 0 : c->nunpd;
+    const int m = s == 0 ? ns : c->ndocc;
+    double* P = c->d_P.p + (size_t)s * ns * ns;
+    if (m == 0) continue;
+    hipLaunchKernelGGL(k_copy_block, dim3((m * m + 255) / 256), dim3(256), 0, st, c->d_Sorb.p, ns, off, m, P);
+    hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), m * sizeof(double), st, P, m, c->d_status.p);
+    HIP_OK(hipGetLastError());
+    // E1 += w tr(h_s P_s): closed shell counts both spins (w = 2)
+    hipLaunchKernelGGL(k_trace_prod, dim3(1), dim3(256), 0, st, c->d_Horb.p, ns, off, P, m, nspin == 1 ? 2.0 : 1.0,
+                       c->d_scal.p + 0);
+    // R_s = Cs_s^T P_s Cs_s
+    const double* Cs = c->d_Cs.p + (size_t)off * n;
+    HIP_OK(gemm(st, m, n, m, 1.0, P, m, false, Cs, n, false, 0.0, c->d_tmp.p, n));
+    HIP_OK(gemm(st, n, n, m, 1.0, Cs, n, true, c->d_tmp.p, n, false, 0.0, c->d_R.p + (size_t)s * n * n, n));
+  }
+  if (nspin == 2 && c->ndocc == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, st, c->d_R.p + (size_t)n * n, (size_t)n * n, 0.0);
+  }
+  return VSVB_OK;
+}
+
+static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
+  hipStream_t st = c->stream;
+  const int nspin = c->nunpd > 0 ? 2 : 1;
+  const size_t nn = (size_t)c->nbf * c->nbf;
+  HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * sizeof(unsigned long long), st));
+  if (jk) {
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_J.p, nn, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_K.p, nspin * nn, 0.0);
+  }
+  HIP_OK(hipEventRecord(c->ev0, st));
+  size_t poff = 0;
+  for (int bc = 0; bc < kNPairClass; ++bc)
+    for (int kc = 0; kc <= bc; ++kc) {
+      const int q = quartet_class_index(bc, kc);
+      QuartetArgs a{};
+      a.pairs = c->d_pairs.p;
+      a.pp = c->d_pp.p;
+      a.shells = c->d_shells.p;
+      a.boys = c->d_boys.p;
+      a.bra0 = c->pair_off[bc];
+      a.nbra = c->pair_off[bc + 1] - a.bra0;
+      a.ket0 = c->pair_off[kc];
+      a.nket = c->pair_off[kc + 1] - a.ket0;
+      a.same = bc == kc;
+      a.ntk = (a.nket + kTile - 1) / kTile;
+      a.ntiles = c->ntiles_class[q];
+      a.tau = c->tau;
+      a.nbf = c->nbf;
+      a.nwfn = 1;
+      a.nspin = nspin;
+      a.dens = c->d_R.p;
+      a.partial = c->d_partial.p + poff;
+      a.jacc = c->d_J.p;
+      a.kacc = c->d_K.p;
+      a.counters = c->d_counters.p;
+      HIP_OK(launch_quartet_class(bc, kc, a, jk, c->blocks_per_class[q], st));
+      poff += c->partial_per_class[q];
+    }
+  HIP_OK(hipEventRecord(c->ev1, st));
+  if (!jk) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, c->d_partial.p, (long)poff, 1, e2_dev);
+    HIP_OK(hipGetLastError());
+  }
+  return VSVB_OK;
+}
+
+static int finish_counters(vsvb_context* c) {
+  unsigned long long h[4];
+  HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_counters[0] = (double)h[0];
+  c->last_counters[1] = (double)h[1];
+  c->last_counters[2] = (double)h[2];
+  c->last_counters[3] = ms;
+  return VSVB_OK;
+}
+
+static int check_status(vsvb_context* c, bool for_grad) {
+  int st[4];
+  HIP_OK(hipMemcpyAsync(st, c->d_status.p, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (st[0]) return fail(VSVB_ERR_NUMERIC, "orbital overlap block is not positive definite (linearly dependent orbitals)");
+  if (for_grad && st[1])
+    return fail(VSVB_ERR_UNSUPPORTED, "gradient requested while the per-shell weight screen drops shells (raise ntol_d)");
+  return VSVB_OK;
+}
+
+static int energy_impl(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* parts) {
+  int rc;
+  HIP_OK(hipSetDevice(c->device));
+  if ((rc = prepare_geometry(c, coords))) return rc;
+  if ((rc = prepare_orbitals(c, coeff, nullptr))) return rc;
+  if ((rc = run_quartets(c, false, c->d_scal.p + 1))) return rc;
+  hipLaunchKernelGGL(k_final_energy, dim3(1), dim3(1), 0, c->stream, c->d_scal.p, c->d_scal.p + 1, c->enuc,
+                     c->d_scal.p + 2);
+  double h[3];
+  HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = check_status(c, false))) return rc;
+  if ((rc = finish_counters(c))) return rc;
+  if (!std::isfinite(h[2])) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
+  if (energy) *energy = h[2];
+  if (parts) {
+    parts[0] = h[0] + h[1];
+    parts[1] = 1.0;
+    parts[2] = h[0];
+    parts[3] = c->enuc;
+  }
+  return VSVB_OK;
+}
+
+// Gradient assembly from the J/K accumulators.
+//   J = A + A^T (Coulomb matrix of RT),  Kx_s = B_s + B_s^T (x_s * exchange matrix of D_s)
+//   F2_s = dE2/dD_s = cfac J - Kx_s,  E2 = 1/2 sum_s tr(D_s F2_s)
+//   F_s = w1 h + F2_s,  dE/dC_s = 2 P_s C_s F_s (1 - R_s S_ao)   (row-major C_s = orbital rows)
+static int gradient_impl(vsvb_context* c, double* grad_dev) {
+  hipStream_t st = c->stream;
+  const int n = c->nbf, ns = c->nsp;
+  const size_t nn = (size_t)n * n;
+  const int nspin = c->nunpd > 0 ? 2 : 1;
+  const int T = 256;
+  const int gb = (int)((nn + T - 1) / T);
+  const double cfac = nspin == 1 ? 2.0 : 1.0, w1 = nspin == 1 ? 2.0 : 1.0;
+  hipLaunchKernelGGL(k_sym_combine, dim3(gb), dim3(T), 0, st, c->d_J.p, 1.0, (const double*)nullptr, 0.0, n,
+                     c->d_Jsym.p);
+  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_gradC.p, (size_t)ns * n, 0.0);
+  for (int s = 0; s < nspin; ++s) {
+    const int off = s == 0 ? 0 : c->nunpd;
+    const int m = s == 0 ? ns : c->ndocc;
+    if (m == 0) continue;
+    double* F = c->d_F.p + (size_t)s * nn;
+    hipLaunchKernelGGL(k_sym_combine, dim3(gb), dim3(T), 0, st, c->d_K.p + (size_t)s * nn, -1.0, c->d_Jsym.p, cfac,
+                       n, F);
+    hipLaunchKernelGGL(k_trace_prod, dim3(1), dim3(256), 0, st, c->d_R.p + (size_t)s * nn, n, 0, F, n, 0.5,
+                       c->d_scal.p + 3);
+    hipLaunchKernelGGL(k_sym_combine, dim3(gb), dim3(T), 0, st, c->d_H.p, 0.5 * w1, F, 1.0, n, F);
+    const double* Ct = c->d_Ct.p + (size_t)off * n;
+    const double* P = c->d_P.p + (size_t)s * ns * ns;
+    HIP_OK(gemm(st, m, n, m, 1.0, P, m, false, Ct, n, false, 0.0, c->d_X.p, n));                   // X = P C
+    HIP_OK(gemm(st, n, n, m, 1.0, Ct, n, true, c->d_X.p, n, false, 0.0, c->d_Rfull.p, n));       // R = C^T P C
+    HIP_OK(gemm(st, m, n, n, 1.0, Ct, n, false, F, n, false, 0.0, c->d_X.p, n));                  // X = C F
+    HIP_OK(gemm(st, m, n, n, 1.0, c->d_X.p, n, false, c->d_Rfull.p, n, false, 0.0, c->d_Z.p, n)); // Z = X R
+    HIP_OK(gemm(st, m, n, n, -1.0, c->d_Z.p, n, false, c->d_S.p, n, false, 1.0, c->d_X.p, n));    // X -= Z S
+    HIP_OK(gemm(st, m, n, m, 2.0, P, m, false, c->d_X.p, n, false, 1.0, c->d_gradC.p + (size_t)off * n, n));
+  }
+  hipLaunchKernelGGL(k_gather_grad, dim3((c->totlen + 255) / 256), dim3(256), 0, st, c->d_ent.p, c->totlen, ns,
+                     c->d_angn.p, c->d_gradC.p, n, grad_dev);
+  HIP_OK(hipGetLastError());
+  return VSVB_OK;
+}
+
+static int energy_grad_impl(vsvb_context* c, const double* coords, const double* coeff_host, const double* coeff_dev,
+                            double* grad_dev) {
+  int rc;
+  HIP_OK(hipSetDevice(c->device));
+  for (int i = 0; i < c->totlen; ++i)
+    if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
+  if ((rc = prepare_geometry(c, coords))) return rc;
+  if ((rc = prepare_orbitals(c, coeff_host, coeff_dev))) return rc;
+  if ((rc = run_quartets(c, true, nullptr))) return rc;
+  return gradient_impl(c, grad_dev);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* vsvb_last_error(void) { return g_last_error.c_str(); }
+
+const char* vsvb_version(void) { return "vsvb-mi355x 0.1 (gfx950, FP64 OS/HGP)"; }
+
+int vsvb_context_create(const vsvb_system* sys, const vsvb_options* opt, vsvb_context** ctx) {
+  if (!ctx) return fail(VSVB_ERR_ARG, "null ctx pointer");
+  *ctx = nullptr;
+  auto* c = new vsvb_context();
+  const int rc = build_context(sys, opt, c);
+  if (rc != VSVB_OK) {
+    vsvb_context_destroy(c);
+    return rc;
+  }
+  *ctx = c;
+  return VSVB_OK;
+}
+
+void vsvb_context_destroy(vsvb_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  hipStream_t st = c->stream;
+  delete c;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+int vsvb_energy(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* parts) {
+  if (!c) return fail(VSVB_ERR_ARG, "null context");
+  return energy_impl(c, coords, coeff, energy, parts);
+}
+
+int vsvb_energy_parts(vsvb_context* c, const double* coeff, double* num, double* wfnorm) {
+  double parts[4];
+  const int rc = vsvb_energy(c, nullptr, coeff, nullptr, parts);
+  if (rc) return rc;
+  if (num) *num = parts[0];
+  if (wfnorm) *wfnorm = parts[1];
+  return VSVB_OK;
+}
+
+int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* grad) {
+  if (!c) return fail(VSVB_ERR_ARG, "null context");
+  int rc = energy_grad_impl(c, coords, coeff, nullptr, c->d_grad.p);
+  if (rc) return rc;
+  double h[4];
+  HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (grad) HIP_OK(hipMemcpyAsync(grad, c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = check_status(c, true))) return rc;
+  if ((rc = finish_counters(c))) return rc;
+  const double e = h[0] + h[3] + c->enuc;
+  if (!std::isfinite(e)) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
+  if (energy) *energy = e;
+  return VSVB_OK;
+}
+
+int vsvb_energy_grad_device(vsvb_context* c, const double* coeff_device, double* energy_device, double* grad_device,
+                            void* stream) {
+  (void)c; (void)coeff_device; (void)energy_device; (void)grad_device; (void)stream;
+  return fail(VSVB_ERR_UNSUPPORTED, "device-pointer gradient not available yet");
+}
+
+int vsvb_debug_buffer(vsvb_context* c, const char* name, double* out, int64_t n) {
+  if (!c || !name || !out) return fail(VSVB_ERR_ARG, "null argument");
+  struct { const char* nm; const DevBuf<double>* b; } tab[] = {
+      {"S_ao", &c->d_S}, {"h_ao", &c->d_H}, {"C", &c->d_Ct}, {"C_eri", &c->d_Cs}, {"S_orb", &c->d_Sorb},
+      {"h_orb", &c->d_Horb}, {"P", &c->d_P}, {"R", &c->d_R}, {"J", &c->d_J}, {"K", &c->d_K}, {"F", &c->d_F},
+      {"gradC", &c->d_gradC}, {"scalars", &c->d_scal}};
+  for (auto& t : tab)
+    if (!std::strcmp(t.nm, name)) {
+      if ((size_t)n > t.b->n) return fail(VSVB_ERR_ARG, "request larger than the buffer");
+      HIP_OK(hipStreamSynchronize(c->stream));
+      HIP_OK(hipMemcpy(out, t.b->p, n * sizeof(double), hipMemcpyDeviceToHost));
+      return VSVB_OK;
+    }
+  return fail(VSVB_ERR_ARG, std::string("unknown buffer ") + name);
+}
+
+int vsvb_last_counters(vsvb_context* c, double* out4) {
+  if (!c || !out4) return fail(VSVB_ERR_ARG, "null argument");
+  std::memcpy(out4, c->last_counters, sizeof(c->last_counters));
+  return VSVB_OK;
+}
+
+int vsvb_batch_create(int32_t, const vsvb_system* const*, const vsvb_options*, vsvb_batch** b) {
+  if (b) *b = nullptr;
+  return fail(VSVB_ERR_UNSUPPORTED, "batched mode not available yet");
+}
+int vsvb_batch_run(vsvb_batch*, int32_t, double*, double* const*) {
+  return fail(VSVB_ERR_UNSUPPORTED, "batched mode not available yet");
+}
+void vsvb_batch_destroy(vsvb_batch*) {}
+
+}  // extern "C"

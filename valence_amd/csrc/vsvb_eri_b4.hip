@@ -1,0 +1,23 @@
+// Instantiates the quartet kernels whose bra pair class is (dp|; one translation unit per bra class so they compile in parallel.
+#include "vsvb_eri_kernels.h"
+
+namespace vsvb {
+
+hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st) {
+  switch (kc) {
+    case 0: return launch_q<2, 1, 0, 0>(a, jk, nblocks, st);
+    case 1: return launch_q<2, 1, 1, 0>(a, jk, nblocks, st);
+    case 2: return launch_q<2, 1, 1, 1>(a, jk, nblocks, st);
+    case 3: return launch_q<2, 1, 2, 0>(a, jk, nblocks, st);
+    case 4: return launch_q<2, 1, 2, 1>(a, jk, nblocks, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_schwarz_b4(Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np, hipStream_t st) {
+  if (np <= 0) return hipSuccess;
+  hipLaunchKernelGGL((schwarz_kernel<2, 1>), dim3((np + 63) / 64), dim3(64), 0, st, pairs, pp, boys, p0, np);
+  return hipGetLastError();
+}
+
+}  // namespace vsvb

@@ -1,0 +1,193 @@
+"""ctypes binding of libvsvb_hip.so (the C ABI in include/vsvb_hip.h).
+
+This is the in-process replacement of the reference's process boundary for
+energies (vtools runs the ``valence`` binary through a pipe,
+vtools/iotools.py:487-495).  The library must have been built in-tree
+(``python -m valence_amd.build``); loading fails loudly if it is missing --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .system import VsvbSystem, pack
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvsvb_hip.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+VSVB_OK = 0
+
+
+class VsvbError(RuntimeError):
+    pass
+
+
+class CSystem(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("natom", "ntype", "nshell_type", "nprim", "norbs", "mxctr", "totlen",
+                                          "npair", "nunpd", "ndocc", "ndf", "nspinc", "ntol_d", "ntol_i")] + [
+        ("atom_type", _i32p), ("coords", _f64p), ("nuc_charge", _f64p),
+        ("type_shell_start", _i32p), ("shell_l", _i32p), ("shell_prim_start", _i32p),
+        ("prim_exp", _f64p), ("prim_coef", _f64p),
+        ("orb_natom", _i32p), ("orb_atoms", _i32p), ("orb_start", _i32p), ("xpset", _i32p),
+        ("coeff", _f64p), ("pair_sc", _i32p), ("coeff_sc", _f64p)]
+
+
+class COptions(C.Structure):
+    _fields_ = [("quartet_tau", C.c_double), ("device", C.c_int32), ("reserved", C.c_int32)]
+
+
+EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_energy_parts", "vsvb_energy_grad",
+           "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
+           "vsvb_last_error", "vsvb_last_counters", "vsvb_debug_buffer", "vsvb_version"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise VsvbError(f"{path} not found: build it with `python -m valence_amd.build` (no CPU fallback exists)")
+    L = C.CDLL(path)
+    L.vsvb_context_create.argtypes = [C.POINTER(CSystem), C.POINTER(COptions), C.POINTER(C.c_void_p)]
+    L.vsvb_context_destroy.argtypes = [C.c_void_p]
+    L.vsvb_context_destroy.restype = None
+    L.vsvb_energy.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+    L.vsvb_energy_parts.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+    L.vsvb_energy_grad.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+    L.vsvb_energy_grad_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vsvb_batch_create.argtypes = [C.c_int32, C.POINTER(C.POINTER(CSystem)), C.POINTER(COptions),
+                                    C.POINTER(C.c_void_p)]
+    L.vsvb_batch_run.argtypes = [C.c_void_p, C.c_int32, _f64p, C.POINTER(_f64p)]
+    L.vsvb_batch_destroy.argtypes = [C.c_void_p]
+    L.vsvb_batch_destroy.restype = None
+    L.vsvb_last_error.restype = C.c_char_p
+    L.vsvb_last_counters.argtypes = [C.c_void_p, _f64p]
+    L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
+    L.vsvb_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray], t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _check(rc: int, what: str):
+    if rc != VSVB_OK:
+        msg = load().vsvb_last_error().decode(errors="replace")
+        raise VsvbError(f"{what} failed ({rc}): {msg}")
+
+
+class _Packed:
+    def __init__(self, s: VsvbSystem):
+        pk = pack(s)
+        self.keep = {**pk.i32, **pk.f64}
+        sc = pk.scalars
+        self.cs = CSystem(**{k: sc[k] for k in ("natom", "ntype", "nshell_type", "nprim", "norbs", "mxctr", "totlen",
+                                                "npair", "nunpd", "ndocc", "ndf", "nspinc", "ntol_d", "ntol_i")},
+                          **{k: _ptr(v, _i32p) for k, v in pk.i32.items()},
+                          **{k: _ptr(v, _f64p) for k, v in pk.f64.items()})
+
+
+class Context:
+    """One molecule/basis/orbital layout resident on the GPU (vsvb_context)."""
+
+    def __init__(self, s: VsvbSystem, quartet_tau: float = 0.0, device: int = 0):
+        L = load()
+        self.sys = s
+        self._packed = _Packed(s)
+        opt = COptions(quartet_tau=quartet_tau, device=device, reserved=0)
+        h = C.c_void_p()
+        _check(L.vsvb_context_create(C.byref(self._packed.cs), C.byref(opt), C.byref(h)), "vsvb_context_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().vsvb_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def energy(self, coeff: Optional[np.ndarray] = None, coords: Optional[np.ndarray] = None,
+               parts: bool = False):
+        cf = None if coeff is None else np.ascontiguousarray(coeff, np.float64)
+        xy = None if coords is None else np.ascontiguousarray(coords, np.float64).reshape(-1)
+        e = C.c_double()
+        p = np.zeros(4)
+        _check(load().vsvb_energy(self._h, _ptr(xy, _f64p), _ptr(cf, _f64p), C.byref(e), _ptr(p, _f64p)),
+               "vsvb_energy")
+        return (e.value, p) if parts else e.value
+
+    def energy_grad(self, coeff: Optional[np.ndarray] = None, coords: Optional[np.ndarray] = None):
+        cf = None if coeff is None else np.ascontiguousarray(coeff, np.float64)
+        xy = None if coords is None else np.ascontiguousarray(coords, np.float64).reshape(-1)
+        e = C.c_double()
+        g = np.zeros(self.sys.totlen)
+        _check(load().vsvb_energy_grad(self._h, _ptr(xy, _f64p), _ptr(cf, _f64p), C.byref(e), _ptr(g, _f64p)),
+               "vsvb_energy_grad")
+        return e.value, g
+
+    def buffer(self, name: str, n: int) -> np.ndarray:
+        """Copy of an internal device buffer (diagnostics)."""
+        out = np.zeros(n)
+        _check(load().vsvb_debug_buffer(self._h, name.encode(), _ptr(out, _f64p), n), "vsvb_debug_buffer")
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(4)
+        _check(load().vsvb_last_counters(self._h, _ptr(out, _f64p)), "vsvb_last_counters")
+        return out
+
+
+class Batch:
+    """Many wave functions of uneven size evaluated in one graph launch (vsvb_batch)."""
+
+    def __init__(self, systems: Sequence[VsvbSystem], quartet_tau: float = 0.0, device: int = 0):
+        L = load()
+        self.systems = list(systems)
+        self._packed = [_Packed(s) for s in self.systems]
+        arr = (C.POINTER(CSystem) * len(self._packed))(*[C.pointer(p.cs) for p in self._packed])
+        self._arr = arr
+        opt = COptions(quartet_tau=quartet_tau, device=device, reserved=0)
+        h = C.c_void_p()
+        _check(L.vsvb_batch_create(len(self.systems), arr, C.byref(opt), C.byref(h)), "vsvb_batch_create")
+        self._h = h
+
+    def run(self, want_grad: bool = True):
+        n = len(self.systems)
+        e = np.zeros(n)
+        grads: List[np.ndarray] = [np.zeros(s.totlen) for s in self.systems]
+        gp = (_f64p * n)(*[_ptr(g, _f64p) for g in grads]) if want_grad else None
+        _check(load().vsvb_batch_run(self._h, int(want_grad), _ptr(e, _f64p), gp), "vsvb_batch_run")
+        return e, (grads if want_grad else None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().vsvb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -82,11 +82,17 @@ class _ListDirectedReader:
 
 
 def _f(tok: str) -> float:
-    return float(tok.replace("d", "e").replace("D", "e"))
+    try:
+        return float(tok.replace("d", "e").replace("D", "e"))
+    except ValueError:
+        raise InputError(f"expected a real number, found {tok!r}") from None
 
 
 def _i(tok: str) -> int:
-    return int(tok)
+    try:
+        return int(tok)
+    except ValueError:
+        raise InputError(f"expected an integer, found {tok!r}") from None
 
 
 @dataclasses.dataclass
