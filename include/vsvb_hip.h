@@ -114,6 +114,16 @@ const char *vsvb_last_error(void);
 /* Counters of the last call: [0] shell quartets evaluated, [1] primitive quartets,
  * [2] shell quartets skipped by the Schwarz cut, [3] device milliseconds of the ERI stage. */
 int vsvb_last_counters(vsvb_context *ctx, double *out4);
+/* Enable per-class HIP-event timing of the shell-quartet launches (off by default). */
+int vsvb_set_profiling(vsvb_context *ctx, int32_t on);
+/* Per quartet class (21 classes, bra pair class major: ss|ss, ps|ss, ps|ps, pp|ss, ...):
+ * out[21][6] = {device ms (profiling on), shell quartets, primitive quartets,
+ * quartets skipped by the Schwarz cut, algorithmic FP64 operations, launches} of the last call. */
+int vsvb_class_stats(vsvb_context *ctx, double *out);
+/* Batch versions: profiling switch for every wave function, and the per-class
+ * statistics summed over the wave functions of the last run. */
+int vsvb_batch_set_profiling(vsvb_batch *batch, int32_t on);
+int vsvb_batch_class_stats(vsvb_batch *batch, double *out);
 /* Copy n doubles of a named internal device buffer to host memory (diagnostics and
  * tests): "S_ao", "h_ao", "C", "C_eri", "S_orb", "h_orb", "P", "R", "J", "K", "F",
  * "gradC", "scalars".  Layouts are row-major; see the implementation for shapes. */

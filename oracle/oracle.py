@@ -40,6 +40,7 @@ class OrcResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("e_nuc", C.c_double), ("e1", C.c_double), ("e2", C.c_double),
                 ("wfnorm", C.c_double), ("ntasks", C.c_longlong), ("tasks_done", C.c_longlong),
                 ("n_schwarz_skip", C.c_longlong), ("n_post_skip", C.c_longlong), ("n_dets", C.c_longlong),
+                ("n_int2e_task", C.c_longlong), ("n_int2e_schwarz", C.c_longlong),
                 ("seconds_setup", C.c_double), ("seconds_1e", C.c_double), ("seconds_2e", C.c_double)]
 
 
@@ -105,6 +106,8 @@ class OracleEnergy:
     n_schwarz_skip: int
     n_post_skip: int
     n_dets: int
+    n_int2e_task: int
+    n_int2e_schwarz: int
     seconds_setup: float
     seconds_1e: float
     seconds_2e: float

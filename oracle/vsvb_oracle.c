@@ -1063,11 +1063,14 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
         }
     Eri eri = {&m, NULL, nsp};
     if (opt->eri_table) eri.G = eri_table(&m);
-    /* schwarz_ints (valence.F90:1489-1523) */
+    /* schwarz_ints (valence.F90:1489-1523).  In the literal sampling mode
+     * (direct int2e, a task slice) values are computed on first use instead. */
     int P = nsp * (nsp + 1) / 2;
     double *schwarz = (double *)calloc(P + 2, sizeof(double));
+    const int lazy = !opt->eri_table && opt->task_end >= 0;
     for (int i = 1; i <= nsp; ++i)
         for (int j = 1; j <= i; ++j) {
+            if (lazy) { schwarz[indx(i, j)] = -1.0; continue; }
             int ie = i > nnd ? 2 * i - nnd : i, je = j > nnd ? 2 * j - nnd : j;
             double g = eri_get(&eri, bra[ie], bra[je], bra[ie], bra[je]);
             schwarz[indx(i, j)] = sqrt(g);
@@ -1109,8 +1112,8 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
     long long tb = opt->task_begin > 0 ? opt->task_begin : 0;
     long long te = opt->task_end >= 0 && opt->task_end < ntasks ? opt->task_end : ntasks;
     double energy2 = 0.0;
-    long long nsk = 0, npo = 0;
-#pragma omp parallel reduction(+ : energy2, nsk, npo, ndet)
+    long long nsk = 0, npo = 0, ncalls = 0, nsch = 0;
+#pragma omp parallel reduction(+ : energy2, nsk, npo, ndet, ncalls, nsch) if (!lazy)
     {
         DWork w;
         dwork_init(&w, s, wdet, dtol);
@@ -1122,6 +1125,16 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
             dtriang(ijorb, &io, &jo);
             dtriang(klorb, &ko, &lo);
             if ((io == jo && io <= nnd) || (ko == lo && ko <= nnd)) continue;
+            if (lazy) {
+                int pr[4][2] = {{io, ko}, {jo, lo}, {io, lo}, {jo, ko}};
+                for (int x = 0; x < 4; ++x) {
+                    int ix = indx(pr[x][0], pr[x][1]);
+                    if (schwarz[ix] < 0.0) {
+                        schwarz[ix] = sqrt(eri_get(&eri, pr[x][0], pr[x][1], pr[x][0], pr[x][1]));
+                        ++nsch;
+                    }
+                }
+            }
             int erep_sig = schwarz[indx(io, ko)] * schwarz[indx(jo, lo)] > itol;
             int exch_sig = schwarz[indx(io, lo)] * schwarz[indx(jo, ko)] > itol;
             double gint_e = 0.0, gint_x = 0.0;
@@ -1132,10 +1145,10 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
                 } else {
                     int ie = io > nnd ? 2 * io - nnd - 1 : io, je = jo > nnd ? 2 * jo - nnd - 1 : jo;
                     int ke = ko > nnd ? 2 * ko - nnd - 1 : ko, le = lo > nnd ? 2 * lo - nnd - 1 : lo;
-                    if (erep_sig) gint_e = eri_get(&eri, bra[ie], bra[ke], bra[je], bra[le]);
+                    if (erep_sig) { gint_e = eri_get(&eri, bra[ie], bra[ke], bra[je], bra[le]); ++ncalls; }
                     if (exch_sig) {
                         if (erep_sig && bra[le] == bra[ke]) gint_x = gint_e;
-                        else gint_x = eri_get(&eri, bra[ie], bra[le], bra[je], bra[ke]);
+                        else { gint_x = eri_get(&eri, bra[ie], bra[le], bra[je], bra[ke]); ++ncalls; }
                     }
                 }
             } else
@@ -1193,6 +1206,8 @@ int orc_energy(const orc_system *s, const orc_options *opt, orc_result *res) {
     res->n_schwarz_skip = nsk;
     res->n_post_skip = npo;
     res->n_dets = ndet;
+    res->n_int2e_task = ncalls;
+    res->n_int2e_schwarz = nsch;
     free(eri.G); free(schwarz); free(wdet); free(So); free(Ho); free(bra);
     mol_free(&m);
     return 0;

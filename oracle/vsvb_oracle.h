@@ -64,6 +64,8 @@ typedef struct {
     long long n_schwarz_skip;  /* tasks with both ERIs Schwarz-screened */
     long long n_post_skip;     /* tasks with both ERIs below itol after evaluation */
     long long n_dets;     /* Givens determinants evaluated */
+    long long n_int2e_task;     /* orbital ERIs evaluated inside the task slice */
+    long long n_int2e_schwarz;  /* orbital ERIs evaluated for Schwarz values */
     double seconds_setup, seconds_1e, seconds_2e;
 } orc_result;
 

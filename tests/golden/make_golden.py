@@ -35,6 +35,8 @@ def main(which):
         data[f"config{i}"] = dict(title=s.title, energy=ref.energy, energy_unscreened=uns.energy,
                                   screen_effect=ref.energy - uns.energy, e_nuc=ref.e_nuc, ntasks=ref.ntasks,
                                   n_schwarz_skip=ref.n_schwarz_skip, n_post_skip=ref.n_post_skip,
+                                  n_int2e_task=ref.n_int2e_task, n_pairs=s.nspatial * (s.nspatial + 1) // 2,
+                                  n_dets=ref.n_dets,
                                   seconds=t_ref, seconds_unscreened=t_uns, threads=os.cpu_count())
         print(f"config{i}: E={ref.energy:.12f} unscreened={uns.energy:.12f} diff={ref.energy - uns.energy:.3e} "
               f"({t_ref:.1f}s, {t_uns:.1f}s)", flush=True)

@@ -97,3 +97,41 @@ def test_gradient_scale_invariance(lib):
         sl = slice(s.orb_start[p], s.orb_start[p + 1])
         v = float(np.dot(s.coeff[sl], g[sl]))
         assert abs(v) < 1e-8 * max(1.0, np.linalg.norm(g[sl]) * np.linalg.norm(s.coeff[sl]))
+
+
+def test_config1_energy(lib):
+    """C8H18 (210 AOs, 33 docc) against the oracle's full energy."""
+    s = workloads.config1()
+    g = CONFIGS["config1"]
+    with lib.Context(s) as ctx:
+        e = ctx.energy()
+        st = ctx.counters()
+    print(f"gpu={e:.12f} oracle={g['energy']:.12f} unscreened={g['energy_unscreened']:.12f} "
+          f"quartets={st[0]:.0f} skipped={st[2]:.0f} eri_ms={st[3]:.2f}")
+    assert abs(e - g["energy_unscreened"]) < 1e-9
+    assert abs(e - g["energy"]) < 1e-9
+
+
+def test_rotation_invariance(lib):
+    """Mixing doubly occupied orbitals leaves a VSHF energy unchanged (size-independent check)."""
+    s = workloads.config1()
+    with lib.Context(s) as ctx:
+        e0 = ctx.energy()
+    r = workloads.rotate_pair(s, 3, 17, 0.04)
+    r = workloads.rotate_pair(r, 0, 30, -0.03)
+    with lib.Context(r) as ctx:
+        e1 = ctx.energy()
+    assert abs(e1 - e0) < 1e-9 * max(1.0, abs(e0) / 1e4)
+
+
+def test_batch_matches_single(lib):
+    systems = workloads.config3()
+    pick = [0, 50, 90, 95]
+    b = lib.Batch([systems[i] for i in pick])
+    e, g = b.run(want_grad=True)
+    for k, i in enumerate(pick):
+        with lib.Context(systems[i]) as ctx:
+            e1, g1 = ctx.energy_grad()
+            e2 = ctx.energy()
+        assert abs(e[k] - e1) < 1e-9 and abs(e[k] - e2) < 1e-9
+        assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)

@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 SOURCES = ["vsvb_core.hip"] + [f"vsvb_eri_b{b}.hip" for b in range(6)]
 ERI_HEADERS = ["vsvb_device.h", "vsvb_eri_kernels.h", "eri_gen.inc"]
-CORE_HEADERS = ERI_HEADERS + [os.path.join("..", "..", "include", "vsvb_hip.h")]
+CORE_HEADERS = ERI_HEADERS + ["eri_flops.inc", os.path.join("..", "..", "include", "vsvb_hip.h")]
 
 # Packed FP32 instructions are disabled for all device code (they have given
 # silent wrong results on this hardware pool); the host pass ignores the flag.

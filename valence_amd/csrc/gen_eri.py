@@ -341,6 +341,18 @@ def main(out_path: str) -> Dict:
         chunks.append("")
         infos.append(info)
     text = "\n".join(chunks) + "\n"
+    # algorithmic FP64 operation counts per class, in launch order (bra class major)
+    flops = ["// Generated by gen_eri.py -- do not edit.",
+             "// FP64 operations per primitive quartet (Boys, VRR, accumulate) and per",
+             "// contracted quartet (HRR), in quartet-class order.",
+             "constexpr double kFlopsPerPrimQuartet[21] = {" +
+             ", ".join(f"{i['flops_per_primitive_quartet']}.0" for i in infos) + "};",
+             "constexpr double kFlopsHrr[21] = {" + ", ".join(f"{i['flops_hrr']}.0" for i in infos) + "};", ""]
+    fpath = os.path.join(os.path.dirname(out_path), "eri_flops.inc")
+    ftext = "\n".join(flops)
+    if not os.path.exists(fpath) or open(fpath).read() != ftext:
+        with open(fpath, "w") as fh:
+            fh.write(ftext)
     if not os.path.exists(out_path) or open(out_path).read() != text:
         with open(out_path, "w") as fh:
             fh.write(text)

@@ -24,6 +24,10 @@
 #include "../../include/vsvb_hip.h"
 #include "vsvb_eri_kernels.h"
 
+namespace vsvb {
+#include "eri_flops.inc"
+}
+
 using namespace vsvb;
 
 // ---------------------------------------------------------------------------
@@ -566,6 +570,7 @@ struct DevBuf {
 struct vsvb_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double tau = 1e-17;
   // sizes
@@ -595,6 +600,11 @@ struct vsvb_context {
   bool geom_ready = false;
   double enuc = 0.0;
   double last_counters[4] = {0, 0, 0, 0};
+  // per-class profiling (events around each class launch when enabled)
+  bool profile = false;
+  bool last_jk = false;
+  hipEvent_t cls_ev[kNQuartetClass + 1] = {};
+  double class_stats[kNQuartetClass][6] = {};
 };
 
 static double nuclear_repulsion(int natom, const double* xyz, const double* Z) {
@@ -654,7 +664,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->device = opt ? opt->device : 0;
   c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
   HIP_OK(hipSetDevice(c->device));
-  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&c->ev0));
   HIP_OK(hipEventCreate(&c->ev1));
   c->natom = s->natom;
@@ -817,7 +827,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(c->d_Z.alloc((size_t)c->nsp * c->nbf));
   HIP_OK(c->d_grad.alloc(std::max(1, s->totlen)));
   HIP_OK(c->d_scal.alloc(16));
-  HIP_OK(c->d_counters.alloc(4));
+  HIP_OK(c->d_counters.alloc(4 * kNQuartetClass));
   // quartet launch geometry and partial buffer
   c->partial_total = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
@@ -928,7 +938,8 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
   hipStream_t st = c->stream;
   const int nspin = c->nunpd > 0 ? 2 : 1;
   const size_t nn = (size_t)c->nbf * c->nbf;
-  HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * sizeof(unsigned long long), st));
+  HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * kNQuartetClass * sizeof(unsigned long long), st));
+  c->last_jk = jk;
   if (jk) {
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_J.p, nn, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_K.p, nspin * nn, 0.0);
@@ -958,10 +969,12 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
       a.partial = c->d_partial.p + poff;
       a.jacc = c->d_J.p;
       a.kacc = c->d_K.p;
-      a.counters = c->d_counters.p;
+      a.counters = c->d_counters.p + 4 * q;
+      if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
       HIP_OK(launch_quartet_class(bc, kc, a, jk, c->blocks_per_class[q], st));
       poff += c->partial_per_class[q];
     }
+  if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));
   HIP_OK(hipEventRecord(c->ev1, st));
   if (!jk) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, c->d_partial.p, (long)poff, 1, e2_dev);
@@ -970,14 +983,39 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
   return VSVB_OK;
 }
 
+// Algorithmic FP64 operations of the density contraction per contracted quartet.
+static double digestion_flops(int q, int nspin, bool jk) {
+  int bc = 0, kc = 0;
+  for (int b = 0; b < kNPairClass; ++b)
+    for (int k = 0; k <= b; ++k)
+      if (quartet_class_index(b, k) == q) { bc = b; kc = k; }
+  const double nab = ncart(kPairLs[bc][0]) * ncart(kPairLs[bc][1]);
+  const double ncd = ncart(kPairLs[kc][0]) * ncart(kPairLs[kc][1]);
+  return jk ? nab * ncd * (4.0 + 8.0 * nspin) : nab * ncd * (2.0 + 4.0 * nspin) + 2.0 * nab;
+}
+
 static int finish_counters(vsvb_context* c) {
-  unsigned long long h[4];
+  unsigned long long h[4 * kNQuartetClass];
   HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  c->last_counters[0] = (double)h[0];
-  c->last_counters[1] = (double)h[1];
-  c->last_counters[2] = (double)h[2];
+  const int nspin = c->nunpd > 0 ? 2 : 1;
+  double tot[3] = {0, 0, 0};
+  for (int q = 0; q < kNQuartetClass; ++q) {
+    for (int k = 0; k < 3; ++k) tot[k] += (double)h[4 * q + k];
+    float cms = 0.0f;
+    if (c->profile && c->ntiles_class[q] > 0) HIP_OK(hipEventElapsedTime(&cms, c->cls_ev[q], c->cls_ev[q + 1]));
+    c->class_stats[q][0] = cms;
+    c->class_stats[q][1] = (double)h[4 * q];
+    c->class_stats[q][2] = (double)h[4 * q + 1];
+    c->class_stats[q][3] = (double)h[4 * q + 2];
+    c->class_stats[q][4] = (double)h[4 * q + 1] * kFlopsPerPrimQuartet[q] +
+                           (double)h[4 * q] * (kFlopsHrr[q] + digestion_flops(q, nspin, c->last_jk));
+    c->class_stats[q][5] = c->ntiles_class[q] > 0 ? 1.0 : 0.0;
+  }
+  c->last_counters[0] = tot[0];
+  c->last_counters[1] = tot[1];
+  c->last_counters[2] = tot[2];
   c->last_counters[3] = ms;
   return VSVB_OK;
 }
@@ -992,14 +1030,20 @@ static int check_status(vsvb_context* c, bool for_grad) {
   return VSVB_OK;
 }
 
-static int energy_impl(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* parts) {
+// Enqueue one plain-energy evaluation on the context stream (no host sync).
+static int enqueue_energy(vsvb_context* c, const double* coords, const double* coeff) {
   int rc;
-  HIP_OK(hipSetDevice(c->device));
   if ((rc = prepare_geometry(c, coords))) return rc;
   if ((rc = prepare_orbitals(c, coeff, nullptr))) return rc;
   if ((rc = run_quartets(c, false, c->d_scal.p + 1))) return rc;
   hipLaunchKernelGGL(k_final_energy, dim3(1), dim3(1), 0, c->stream, c->d_scal.p, c->d_scal.p + 1, c->enuc,
                      c->d_scal.p + 2);
+  HIP_OK(hipGetLastError());
+  return VSVB_OK;
+}
+
+static int finish_energy(vsvb_context* c, double* energy, double* parts) {
+  int rc;
   double h[3];
   HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   if ((rc = check_status(c, false))) return rc;
@@ -1013,6 +1057,13 @@ static int energy_impl(vsvb_context* c, const double* coords, const double* coef
     parts[3] = c->enuc;
   }
   return VSVB_OK;
+}
+
+static int energy_impl(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* parts) {
+  int rc;
+  HIP_OK(hipSetDevice(c->device));
+  if ((rc = enqueue_energy(c, coords, coeff))) return rc;
+  return finish_energy(c, energy, parts);
 }
 
 // Gradient assembly from the J/K accumulators.
@@ -1055,10 +1106,9 @@ static int gradient_impl(vsvb_context* c, double* grad_dev) {
   return VSVB_OK;
 }
 
-static int energy_grad_impl(vsvb_context* c, const double* coords, const double* coeff_host, const double* coeff_dev,
-                            double* grad_dev) {
+static int enqueue_energy_grad(vsvb_context* c, const double* coords, const double* coeff_host,
+                               const double* coeff_dev, double* grad_dev) {
   int rc;
-  HIP_OK(hipSetDevice(c->device));
   for (int i = 0; i < c->totlen; ++i)
     if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
   if ((rc = prepare_geometry(c, coords))) return rc;
@@ -1066,6 +1116,31 @@ static int energy_grad_impl(vsvb_context* c, const double* coords, const double*
   if ((rc = run_quartets(c, true, nullptr))) return rc;
   return gradient_impl(c, grad_dev);
 }
+
+static int finish_energy_grad(vsvb_context* c, double* energy, double* grad) {
+  int rc;
+  double h[4];
+  HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (grad) HIP_OK(hipMemcpyAsync(grad, c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = check_status(c, true))) return rc;
+  if ((rc = finish_counters(c))) return rc;
+  const double e = h[0] + h[3] + c->enuc;
+  if (!std::isfinite(e)) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
+  if (energy) *energy = e;
+  return VSVB_OK;
+}
+
+// First batched mode: one context per wave function, every evaluation enqueued
+// on its own stream before any host synchronization, so independent wave
+// functions overlap on the device.
+struct vsvb_batch {
+  std::vector<vsvb_context*> ctx;
+  hipStream_t stream = nullptr;
+  ~vsvb_batch() {
+    for (auto* c : ctx) vsvb_context_destroy(c);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1095,7 +1170,9 @@ void vsvb_context_destroy(vsvb_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  hipStream_t st = c->stream;
+  for (int q = 0; q <= kNQuartetClass; ++q)
+    if (c->cls_ev[q]) (void)hipEventDestroy(c->cls_ev[q]);
+  hipStream_t st = c->own_stream ? c->stream : nullptr;
   delete c;
   if (st) (void)hipStreamDestroy(st);
 }
@@ -1116,17 +1193,10 @@ int vsvb_energy_parts(vsvb_context* c, const double* coeff, double* num, double*
 
 int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* grad) {
   if (!c) return fail(VSVB_ERR_ARG, "null context");
-  int rc = energy_grad_impl(c, coords, coeff, nullptr, c->d_grad.p);
+  HIP_OK(hipSetDevice(c->device));
+  int rc = enqueue_energy_grad(c, coords, coeff, nullptr, c->d_grad.p);
   if (rc) return rc;
-  double h[4];
-  HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  if (grad) HIP_OK(hipMemcpyAsync(grad, c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if ((rc = check_status(c, true))) return rc;
-  if ((rc = finish_counters(c))) return rc;
-  const double e = h[0] + h[3] + c->enuc;
-  if (!std::isfinite(e)) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
-  if (energy) *energy = e;
-  return VSVB_OK;
+  return finish_energy_grad(c, energy, grad);
 }
 
 int vsvb_energy_grad_device(vsvb_context* c, const double* coeff_device, double* energy_device, double* grad_device,
@@ -1151,19 +1221,90 @@ int vsvb_debug_buffer(vsvb_context* c, const char* name, double* out, int64_t n)
   return fail(VSVB_ERR_ARG, std::string("unknown buffer ") + name);
 }
 
+int vsvb_set_profiling(vsvb_context* c, int32_t on) {
+  if (!c) return fail(VSVB_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  if (on && !c->cls_ev[0])
+    for (int q = 0; q <= kNQuartetClass; ++q) HIP_OK(hipEventCreate(&c->cls_ev[q]));
+  c->profile = on != 0;
+  return VSVB_OK;
+}
+
+int vsvb_batch_set_profiling(vsvb_batch* b, int32_t on) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  for (auto* c : b->ctx) {
+    const int rc = vsvb_set_profiling(c, on);
+    if (rc) return rc;
+  }
+  return VSVB_OK;
+}
+
+/* Sum of the per-class statistics of every wave function of the last run. */
+int vsvb_batch_class_stats(vsvb_batch* b, double* out) {
+  if (!b || !out) return fail(VSVB_ERR_ARG, "null argument");
+  std::memset(out, 0, sizeof(double) * kNQuartetClass * 6);
+  for (auto* c : b->ctx)
+    for (int q = 0; q < kNQuartetClass; ++q)
+      for (int k = 0; k < 6; ++k) out[q * 6 + k] += c->class_stats[q][k];
+  return VSVB_OK;
+}
+
+int vsvb_class_stats(vsvb_context* c, double* out) {
+  if (!c || !out) return fail(VSVB_ERR_ARG, "null argument");
+  std::memcpy(out, c->class_stats, sizeof(c->class_stats));
+  return VSVB_OK;
+}
+
 int vsvb_last_counters(vsvb_context* c, double* out4) {
   if (!c || !out4) return fail(VSVB_ERR_ARG, "null argument");
   std::memcpy(out4, c->last_counters, sizeof(c->last_counters));
   return VSVB_OK;
 }
 
-int vsvb_batch_create(int32_t, const vsvb_system* const*, const vsvb_options*, vsvb_batch** b) {
-  if (b) *b = nullptr;
-  return fail(VSVB_ERR_UNSUPPORTED, "batched mode not available yet");
+int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsvb_options* opt, vsvb_batch** b) {
+  if (!b || !systems || nwfn < 1) return fail(VSVB_ERR_ARG, "invalid batch arguments");
+  *b = nullptr;
+  auto* bt = new vsvb_batch();
+  HIP_OK(hipSetDevice(opt ? opt->device : 0));
+  HIP_OK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+  for (int i = 0; i < nwfn; ++i) {
+    auto* c = new vsvb_context();
+    c->stream = bt->stream;
+    c->own_stream = false;
+    const int rc = build_context(systems[i], opt, c);
+    if (rc) {
+      g_last_error = "wave function " + std::to_string(i) + ": " + g_last_error;
+      vsvb_context_destroy(c);
+      delete bt;
+      return rc;
+    }
+    bt->ctx.push_back(c);
+  }
+  *b = bt;
+  return VSVB_OK;
 }
-int vsvb_batch_run(vsvb_batch*, int32_t, double*, double* const*) {
-  return fail(VSVB_ERR_UNSUPPORTED, "batched mode not available yet");
+
+int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* const* grads) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  const size_t n = b->ctx.size();
+  for (size_t i = 0; i < n; ++i) {
+    vsvb_context* c = b->ctx[i];
+    HIP_OK(hipSetDevice(c->device));
+    // geometry-dependent data (pair data, one-electron integrals, Schwarz bounds)
+    // are rebuilt every run: passing the coordinates forces it.
+    const int rc = want_grad ? enqueue_energy_grad(c, c->coords.data(), nullptr, nullptr, c->d_grad.p)
+                             : enqueue_energy(c, c->coords.data(), nullptr);
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    vsvb_context* c = b->ctx[i];
+    const int rc = want_grad ? finish_energy_grad(c, energies ? energies + i : nullptr, grads ? grads[i] : nullptr)
+                             : finish_energy(c, energies ? energies + i : nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return VSVB_OK;
 }
-void vsvb_batch_destroy(vsvb_batch*) {}
+
+void vsvb_batch_destroy(vsvb_batch* b) { delete b; }
 
 }  // extern "C"

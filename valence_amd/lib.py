@@ -45,7 +45,8 @@ class COptions(C.Structure):
 
 EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_energy_parts", "vsvb_energy_grad",
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
-           "vsvb_last_error", "vsvb_last_counters", "vsvb_debug_buffer", "vsvb_version"]
+           "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
+           "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_version"]
 
 _lib = None
 
@@ -73,6 +74,10 @@ def load(path: str = LIB_PATH):
     L.vsvb_last_error.restype = C.c_char_p
     L.vsvb_last_counters.argtypes = [C.c_void_p, _f64p]
     L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
+    L.vsvb_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_class_stats.argtypes = [C.c_void_p, _f64p]
+    L.vsvb_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_batch_class_stats.argtypes = [C.c_void_p, _f64p]
     L.vsvb_version.restype = C.c_char_p
     _lib = L
     return L
@@ -153,6 +158,15 @@ class Context:
         _check(load().vsvb_debug_buffer(self._h, name.encode(), _ptr(out, _f64p), n), "vsvb_debug_buffer")
         return out
 
+    def set_profiling(self, on: bool = True):
+        _check(load().vsvb_set_profiling(self._h, int(on)), "vsvb_set_profiling")
+
+    def class_stats(self) -> np.ndarray:
+        """[21, 6]: ms, quartets, primitive quartets, skipped, algorithmic FP64 ops, launches (last call)."""
+        out = np.zeros((21, 6))
+        _check(load().vsvb_class_stats(self._h, _ptr(out, _f64p)), "vsvb_class_stats")
+        return out
+
     def counters(self) -> np.ndarray:
         out = np.zeros(4)
         _check(load().vsvb_last_counters(self._h, _ptr(out, _f64p)), "vsvb_last_counters")
@@ -180,6 +194,14 @@ class Batch:
         gp = (_f64p * n)(*[_ptr(g, _f64p) for g in grads]) if want_grad else None
         _check(load().vsvb_batch_run(self._h, int(want_grad), _ptr(e, _f64p), gp), "vsvb_batch_run")
         return e, (grads if want_grad else None)
+
+    def set_profiling(self, on: bool = True):
+        _check(load().vsvb_batch_set_profiling(self._h, int(on)), "vsvb_batch_set_profiling")
+
+    def class_stats(self) -> np.ndarray:
+        out = np.zeros((21, 6))
+        _check(load().vsvb_batch_class_stats(self._h, _ptr(out, _f64p)), "vsvb_batch_class_stats")
+        return out
 
     def close(self):
         if getattr(self, "_h", None):
