@@ -78,12 +78,26 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0, help="plain-leg steps (default: 4*steps)")
+    ap.add_argument("--class-stats", default="", help="write per-class statistics (JSON) to this path")
+    ap.add_argument("--leg", choices=["both", "plain", "batch", "copy"], default="both",
+                    help="profiling aid: run one leg only (no JSON line unless both); 'copy' runs a 1 GiB "
+                         "device copy (torch) used to calibrate the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU; the path shards by "
               "whole wave functions across GPUs (see DESIGN.md) and no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.leg == "copy":
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 3 x 1 GiB")
+        return
 
     from valence_amd import lib as L
     from valence_amd import workloads as W
@@ -93,6 +107,8 @@ def main():
     systems = W.config3()
     nplain = args.plain_steps or 4 * args.steps
 
+    if args.leg == "batch":
+        nplain = 0
     # ---- plain leg: one C8H18 energy per step
     ctx = L.Context(plain)
     ctx.set_profiling(True)
@@ -104,7 +120,13 @@ def main():
         e_plain = ctx.energy(plain.coeff, plain.coords)
         stats_plain += ctx.class_stats()
     t_plain = time.perf_counter() - t0
-    plain_eps = nplain / t_plain
+    plain_eps = nplain / t_plain if nplain else 0.0
+    if args.class_stats:
+        json.dump({"classes": CLASS_NAMES, "columns": ["ms", "quartets", "prim_quartets", "skipped", "flops", "launches"],
+                   "plain_per_call": (stats_plain / max(nplain, 1)).tolist()}, open(args.class_stats, "w"))
+    if args.leg == "plain":
+        print(f"plain: {nplain} energies, {1e3 * t_plain / max(nplain, 1):.3f} ms each")
+        return
 
     # ---- batched leg: 96 wave functions, energy + gradient
     batch = L.Batch(systems)
@@ -118,6 +140,9 @@ def main():
         stats_batch += batch.class_stats()
     t_batch = time.perf_counter() - t0
     value = len(systems) * args.steps / t_batch
+    if args.leg == "batch":
+        print(f"batch: {args.steps} runs, {1e3 * t_batch / args.steps:.1f} ms each")
+        return
 
     # ---- roofline of the dominant kernel (largest summed device time, both legs)
     tot = stats_plain + stats_batch

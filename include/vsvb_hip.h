@@ -128,6 +128,9 @@ int vsvb_batch_class_stats(vsvb_batch *batch, double *out);
  * tests): "S_ao", "h_ao", "C", "C_eri", "S_orb", "h_orb", "P", "R", "J", "K", "F",
  * "gradC", "scalars".  Layouts are row-major; see the implementation for shapes. */
 int vsvb_debug_buffer(vsvb_context *ctx, const char *name, double *out, int64_t n);
+/* Evaluate the device Boys function F_0..F_L at n arguments T (diagnostics/tests):
+ * out[i*(L+1) + m] = F_m(T[i]), 0 <= L <= 8. */
+int vsvb_debug_boys(int32_t L, const double *T, int32_t n, double *out);
 /* Library version string. */
 const char *vsvb_version(void);
 

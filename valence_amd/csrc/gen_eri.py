@@ -287,11 +287,12 @@ def gen_class(la, lb, lc, ld):
         em.emit("      const double PAx = B.PAx, PAy = B.PAy, PAz = B.PAz, oo2p = B.oo2p;")
     em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
     em.emit("        const PrimPair K = kp[ik];")
-    em.emit("        const double s = B.p + K.p, rs = 1.0 / s;")
+    em.emit("        const double s = B.p + K.p;")
+    em.emit("        const double r = rsqrt_fp64(s), rs = r * r;")
     em.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")
     em.emit("        const double rho = B.p * K.p * rs;")
     em.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    em.emit("        const double pref = B.K * K.K * sqrt(rs);")
+    em.emit("        const double pref = B.K * K.K * r;")
     em.emit(f"        double Fb[{L + 1}];")
     em.emit(f"        boys_eval<{L}>(T, btab, Fb);")
     for m in range(L + 1):

@@ -358,20 +358,14 @@ __global__ void k_ndf_scale(const OrbEntry* __restrict__ ent, const int* __restr
   scale[f] = 1.0 / sqrt(s);
 }
 
-// Effective weights Ct[p][mu] (angn folded) and the ERI weights Cs with the
-// reference's per-(orbital, shell) screen: shell kept iff sum of its squared
-// normalized raw weights > dtol (int2e, valence.F90:3296-3300).
+// Effective weights Ct[p][mu] (angn folded, NDFs expanded): one thread per orbital.
 __global__ void k_build_orbitals(const OrbEntry* __restrict__ ent, const int* __restrict__ orb_start,
                                  const double* __restrict__ coeff, const double* __restrict__ angn,
-                                 const double* __restrict__ ndf_scale, int first_ndf, const double* __restrict__ S,
-                                 const int* __restrict__ ao_shell, const Shell* __restrict__ shells, int nbf,
-                                 int nsp, double dtol, double* __restrict__ Ct, double* __restrict__ Cs,
-                                 int* __restrict__ nmasked) {
+                                 const double* __restrict__ ndf_scale, int first_ndf, int nbf, int nsp,
+                                 double* __restrict__ Ct) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nsp) return;
   double* c = Ct + (size_t)p * nbf;
-  double* cs = Cs + (size_t)p * nbf;
-  for (int mu = 0; mu < nbf; ++mu) c[mu] = 0.0;
   for (int k = orb_start[p]; k < orb_start[p + 1]; ++k) {
     const OrbEntry e = ent[k];
     if (e.kind == 0) {
@@ -382,33 +376,33 @@ __global__ void k_build_orbitals(const OrbEntry* __restrict__ ent, const int* __
       for (int l = orb_start[f]; l < orb_start[f + 1]; ++l) c[ent[l].tgt] += angn[ent[l].tgt] * coeff[l] * w;
     }
   }
-  // self overlap over the nonzero support
-  double spp = 0.0;
-  for (int mu = 0; mu < nbf; ++mu) {
-    if (c[mu] == 0.0) continue;
-    const double* row = S + (size_t)mu * nbf;
-    for (int nu = 0; nu < nbf; ++nu)
-      if (c[nu] != 0.0) spp += c[mu] * c[nu] * row[nu];
+}
+
+// The reference's per-(orbital, shell) weight screen for the ERIs (int2e,
+// valence.F90:3296-3300): the shell is used only if the sum of its squared
+// weights, after the orbital is normalized (normal, :2157-2182), exceeds dtol.
+// One thread per (orbital, shell); S_pp from the orbital overlap matrix.
+__global__ void k_shell_screen(const double* __restrict__ Ct, const double* __restrict__ Sorb, int nsp,
+                               const Shell* __restrict__ shells, int nsh, const double* __restrict__ angn, int nbf,
+                               double dtol, double* __restrict__ Cs, int* __restrict__ nmasked) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nsp * nsh) return;
+  const int p = (int)(t / nsh), sh = (int)(t % nsh);
+  const Shell S = shells[sh];
+  const double* c = Ct + (size_t)p * nbf + S.ao;
+  double* cs = Cs + (size_t)p * nbf + S.ao;
+  const double inv = 1.0 / Sorb[(size_t)p * nsp + p];
+  const int n = ncart(S.l);
+  double sum = 0.0;
+  bool nz = false;
+  for (int k = 0; k < n; ++k) {
+    const double raw = c[k] / angn[S.ao + k];
+    sum += raw * raw * inv;
+    nz |= c[k] != 0.0;
   }
-  const double inv = 1.0 / spp;
-  int masked = 0;
-  int mu = 0;
-  while (mu < nbf) {
-    const Shell sh = shells[ao_shell[mu]];
-    const int n = ncart(sh.l);
-    double sum = 0.0;
-    bool nz = false;
-    for (int k = 0; k < n; ++k) {
-      const double raw = c[mu + k] / angn[mu + k];
-      sum += raw * raw * inv;
-      nz |= c[mu + k] != 0.0;
-    }
-    const bool keep = sum > dtol;
-    if (nz && !keep) ++masked;
-    for (int k = 0; k < n; ++k) cs[mu + k] = keep ? c[mu + k] : 0.0;
-    mu += n;
-  }
-  if (masked) atomicAdd(nmasked, masked);
+  const bool keep = sum > dtol;
+  for (int k = 0; k < n; ++k) cs[k] = keep ? c[k] : 0.0;
+  if (nz && !keep) atomicAdd(nmasked, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -593,6 +587,8 @@ struct vsvb_context {
   DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
   DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
   DevBuf<unsigned long long> d_counters;
+  DevBuf<int> d_tilectr;
+  int blocks_jk[kNQuartetClass] = {0};
   size_t partial_per_class[kNQuartetClass] = {0};
   int blocks_per_class[kNQuartetClass] = {0};
   int ntiles_class[kNQuartetClass] = {0};
@@ -640,6 +636,18 @@ static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, boo
     case 5: return launch_quartet_b5(kc, a, jk, nblocks, st);
   }
   return hipErrorInvalidValue;
+}
+
+static int class_occupancy(int bc, int kc, bool jk) {
+  switch (bc) {
+    case 0: return occupancy_b0(kc, jk);
+    case 1: return occupancy_b1(kc, jk);
+    case 2: return occupancy_b2(kc, jk);
+    case 3: return occupancy_b3(kc, jk);
+    case 4: return occupancy_b4(kc, jk);
+    case 5: return occupancy_b5(kc, jk);
+  }
+  return 1;
 }
 
 static hipError_t launch_schwarz(int pc, Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np,
@@ -718,7 +726,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->nsh = (int)shells.size();
   c->nbf = ao;
   // shell pairs: class-major, by decreasing primitive-pair count
-  struct HP { Pair p; int cls; };
+  struct HP { Pair p; int cls; double r2; };
   std::vector<HP> hp;
   for (int M = 0; M < c->nsh; ++M)
     for (int N = 0; N <= M; ++N) {
@@ -730,11 +738,15 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       h.p.npp = shells[A].nprim * shells[B].nprim;
       h.p.offdiag = M != N;
       h.cls = pair_class_of(shells[A].l, shells[B].l);
+      const double* xa = s->coords + 3 * shells[A].atom;
+      const double* xb = s->coords + 3 * shells[B].atom;
+      h.r2 = (xa[0] - xb[0]) * (xa[0] - xb[0]) + (xa[1] - xb[1]) * (xa[1] - xb[1]) + (xa[2] - xb[2]) * (xa[2] - xb[2]);
       hp.push_back(h);
     }
   std::stable_sort(hp.begin(), hp.end(), [](const HP& x, const HP& y) {
     if (x.cls != y.cls) return x.cls < y.cls;
-    return x.p.npp > y.p.npp;
+    if (x.p.npp != y.p.npp) return x.p.npp > y.p.npp;
+    return x.r2 < y.r2;
   });
   std::vector<Pair> pairs(hp.size());
   std::vector<PPSrc> ppsrc;
@@ -828,17 +840,22 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(c->d_grad.alloc(std::max(1, s->totlen)));
   HIP_OK(c->d_scal.alloc(16));
   HIP_OK(c->d_counters.alloc(4 * kNQuartetClass));
-  // quartet launch geometry and partial buffer
+  // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
+  int ncu = 0;
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  HIP_OK(c->d_tilectr.alloc(kNQuartetClass));
   c->partial_total = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
       const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
-      const int tb = (nb + kTile - 1) / kTile, tk = (nk + kTile - 1) / kTile;
-      const int nt = (bc == kc) ? tb * (tb + 1) / 2 : tb * tk;
-      c->ntiles_class[q] = (nb > 0 && nk > 0) ? nt : 0;
-      c->blocks_per_class[q] = std::min(c->ntiles_class[q], 2048);
-      c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;
+      const int tb = (nb + kTileB - 1) / kTileB;
+      c->ntiles_class[q] = (nb > 0 && nk > 0) ? tb * nk : 0;  // work units, one wavefront each
+      const int waves = (c->ntiles_class[q] + 3) / 4;
+      // Low-AM energy kernels measured fastest with at most 2048 workgroups (8192 wavefronts).
+      c->blocks_per_class[q] = std::min(waves, std::min(2048, class_occupancy(bc, kc, false) * ncu));
+      c->blocks_jk[q] = std::min(waves, class_occupancy(bc, kc, true) * ncu);
+      c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;  // energy mode writes every entry
       c->partial_total += c->partial_per_class[q];
     }
   HIP_OK(c->d_partial.alloc(std::max<size_t>(c->partial_total, 1) * 2));
@@ -902,15 +919,21 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
                        c->d_coeff.p, c->d_angn.p, c->d_S.p, n, c->first_ndf, c->ndf, c->d_ndf_scale.p);
     HIP_OK(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, st, c->d_Ct.p, (size_t)ns * n, 0.0);
   hipLaunchKernelGGL(k_build_orbitals, dim3((ns + 63) / 64), dim3(64), 0, st, c->d_ent.p, c->d_orb_start.p,
-                     c->d_coeff.p, c->d_angn.p, c->d_ndf_scale.p, c->first_ndf, c->d_S.p, c->d_ao_shell.p,
-                     c->d_shells.p, n, ns, c->dtol, c->d_Ct.p, c->d_Cs.p, c->d_status.p + 1);
+                     c->d_coeff.p, c->d_angn.p, c->d_ndf_scale.p, c->first_ndf, n, ns, c->d_Ct.p);
   HIP_OK(hipGetLastError());
   // S_orb = Ct S Ct^T, h_orb = Ct H Ct^T
   HIP_OK(gemm(st, ns, n, n, 1.0, c->d_Ct.p, n, false, c->d_S.p, n, false, 0.0, c->d_tmp.p, n));
   HIP_OK(gemm(st, ns, ns, n, 1.0, c->d_tmp.p, n, false, c->d_Ct.p, n, true, 0.0, c->d_Sorb.p, ns));
   HIP_OK(gemm(st, ns, n, n, 1.0, c->d_Ct.p, n, false, c->d_H.p, n, false, 0.0, c->d_tmp.p, n));
   HIP_OK(gemm(st, ns, ns, n, 1.0, c->d_tmp.p, n, false, c->d_Ct.p, n, true, 0.0, c->d_Horb.p, ns));
+  {
+    const long nt = (long)ns * c->nsh;
+    hipLaunchKernelGGL(k_shell_screen, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, c->d_Ct.p, c->d_Sorb.p,
+                       ns, c->d_shells.p, c->nsh, c->d_angn.p, n, c->dtol, c->d_Cs.p, c->d_status.p + 1);
+    HIP_OK(hipGetLastError());
+  }
   const int nspin = c->nunpd > 0 ? 2 : 1;
   for (int s = 0; s < nspin; ++s) {
     const int off = s == 0 ? 0 : c->nunpd;
@@ -939,6 +962,7 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
   const int nspin = c->nunpd > 0 ? 2 : 1;
   const size_t nn = (size_t)c->nbf * c->nbf;
   HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * kNQuartetClass * sizeof(unsigned long long), st));
+  HIP_OK(hipMemsetAsync(c->d_tilectr.p, 0, kNQuartetClass * sizeof(int), st));
   c->last_jk = jk;
   if (jk) {
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_J.p, nn, 0.0);
@@ -959,7 +983,7 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
       a.ket0 = c->pair_off[kc];
       a.nket = c->pair_off[kc + 1] - a.ket0;
       a.same = bc == kc;
-      a.ntk = (a.nket + kTile - 1) / kTile;
+      a.ntk = (a.nket + kTileK - 1) / kTileK;
       a.ntiles = c->ntiles_class[q];
       a.tau = c->tau;
       a.nbf = c->nbf;
@@ -970,8 +994,9 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
       a.jacc = c->d_J.p;
       a.kacc = c->d_K.p;
       a.counters = c->d_counters.p + 4 * q;
+      a.tile_ctr = c->d_tilectr.p + q;
       if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
-      HIP_OK(launch_quartet_class(bc, kc, a, jk, c->blocks_per_class[q], st));
+      HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q], st));
       poff += c->partial_per_class[q];
     }
   if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));
@@ -1142,10 +1167,41 @@ struct vsvb_batch {
   }
 };
 
+// Device Boys function at given arguments (diagnostics / tests).
+template <int L>
+__global__ void k_boys_probe(const double* __restrict__ T, int n, const double* __restrict__ tab, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double F[L + 1];
+  boys_eval<L>(T[i], tab, F);
+#pragma unroll
+  for (int m = 0; m <= L; ++m) out[(size_t)i * (L + 1) + m] = F[m];
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
+  if (L < 0 || L > kLTot || n < 1 || !T || !out) return fail(VSVB_ERR_ARG, "invalid Boys probe arguments");
+  auto tab = make_boys_table();
+  DevBuf<double> dt, dT, dout;
+  HIP_OK(dt.alloc(tab.size()));
+  HIP_OK(dT.alloc(n));
+  HIP_OK(dout.alloc((size_t)n * (L + 1)));
+  HIP_OK(hipMemcpy(dt.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dT.p, T, n * sizeof(double), hipMemcpyHostToDevice));
+  const dim3 g((n + 63) / 64), b(64);
+  switch (L) {
+#define BP(LL) case LL: hipLaunchKernelGGL(k_boys_probe<LL>, g, b, 0, 0, dT.p, n, dt.p, dout.p); break;
+    BP(0) BP(1) BP(2) BP(3) BP(4) BP(5) BP(6) BP(7) BP(8)
+#undef BP
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dout.p, (size_t)n * (L + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  return VSVB_OK;
+}
 
 const char* vsvb_last_error(void) { return g_last_error.c_str(); }
 

@@ -19,13 +19,12 @@ constexpr int kLTot = 4 * kLMax;    // highest Boys order for (dd|dd)
 constexpr int kNPairClass = 6;
 constexpr int kNQuartetClass = 21;
 
-// Boys table: F_k(T_i) for T_i = i*dT, i < kBoysN, k < kBoysK (k = L + 0..7).
+// Boys table: F_k(T_i) for T_i = i*dT, i < kBoysN, k < kBoysK (k = L + 0..8).
 constexpr double kBoysDT = 0.05;
 constexpr double kBoysInvDT = 20.0;
-constexpr double kBoysTMax = 36.0;  // above: asymptotic F_0 and upward recursion
-constexpr int kBoysN = 722;
-constexpr int kBoysK = kLTot + 8;
-constexpr int kBoysTerms = 8;
+constexpr double kBoysTMax = 60.0;  // above: asymptotic F_0, upward recursion (e^-T < 1e-26 is dropped)
+constexpr int kBoysN = 1202;
+constexpr int kBoysK = kLTot + 9;
 
 struct PrimPair {
   double p, oo2p, Px, Py, Pz, PAx, PAy, PAz, K, pad;
@@ -44,39 +43,54 @@ struct Pair {
   double Q;        // sqrt(max |(ab|ab)|)
 };
 
-// Boys function F_m(T), m = 0..L.  For T < kBoysTMax: Taylor expansion of
-// F_L about the nearest grid point (F_m' = -F_{m+1}), 8 terms, then the
-// downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).  Above: F_0 =
-// sqrt(pi/T)/2 (erfc(6) < 3e-17) and the upward recursion, stable for T > L.
+// 1/sqrt(x) to full FP64 precision: hardware estimate plus two Newton steps
+// (y <- y + y/2 (1 - x y^2)); x is a positive normal number here.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  return y;
+}
+
+// Boys function F_m(T), m = 0..L, without divisions or exp().
+// T < kBoysTMax: 8-term Taylor expansions of F_L and F_{L+1} about the nearest
+// grid point (dF_m/dT = -F_{m+1}, |T - T_i| <= dT/2, remainder < 4e-18 F),
+// e^{-T} = (2L+1) F_L - 2T F_{L+1} (from the recursion; its absolute error
+// ~eps (2L+1) F_L is damped by the downward steps), then the stable downward
+// recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
+// T >= kBoysTMax: F_0 = sqrt(pi/T)/2 (erfc(sqrt 60) ~ 1e-28) and the upward
+// recursion F_{m+1} = (2m+1) F_m / (2T) (the e^{-T} term is below 1e-26).
+// The two paths branch: a wavefront whose lanes all take one path skips the
+// other entirely, which measured faster than evaluating both on every lane.
 template <int L>
 __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ tab, double* F) {
   if (T < kBoysTMax) {
     const int i = __double2int_rn(T * kBoysInvDT);
     const double d = i * kBoysDT - T;
     const double* t = tab + i * kBoysK + L;
-    double f = t[7];
-    f = t[6] + f * d * (1.0 / 7.0);
-    f = t[5] + f * d * (1.0 / 6.0);
-    f = t[4] + f * d * (1.0 / 5.0);
-    f = t[3] + f * d * (1.0 / 4.0);
-    f = t[2] + f * d * (1.0 / 3.0);
-    f = t[1] + f * d * (1.0 / 2.0);
-    f = t[0] + f * d;
+    double f = t[7], g = t[8];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) {
+      const double dk = d * (1.0 / (k + 1));
+      f = __builtin_fma(f, dk, t[k]);
+      if constexpr (L > 0) g = __builtin_fma(g, dk, t[k + 1]);
+    }
     F[L] = f;
     if constexpr (L > 0) {
-      const double e = exp(-T);
       const double t2 = 2.0 * T;
+      const double e = __builtin_fma(-t2, g, (2 * L + 1) * f);
 #pragma unroll
-      for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+      for (int m = L - 1; m >= 0; --m) F[m] = __builtin_fma(t2, F[m + 1], e) * (1.0 / (2 * m + 1));
     }
   } else {
-    const double rT = 1.0 / T;
-    F[0] = 0.88622692545275801365 * sqrt(rT);  // sqrt(pi)/2
+    const double r = rsqrt_fp64(T);
+    F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
     if constexpr (L > 0) {
-      const double e = exp(-T);
-      const double h = 0.5 * rT;
+      const double h = 0.5 * r * r;
 #pragma unroll
-      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * h) * F[m];
     }
   }
 }

@@ -12,9 +12,18 @@ hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, int nblocks,
   }
 }
 
+int occupancy_b2(int kc, bool jk) {
+  switch (kc) {
+    case 0: return occupancy_q<1, 1, 0, 0>(jk);
+    case 1: return occupancy_q<1, 1, 1, 0>(jk);
+    case 2: return occupancy_q<1, 1, 1, 1>(jk);
+    default: return 1;
+  }
+}
+
 hipError_t launch_schwarz_b2(Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np, hipStream_t st) {
   if (np <= 0) return hipSuccess;
-  hipLaunchKernelGGL((schwarz_kernel<1, 1>), dim3((np + 63) / 64), dim3(64), 0, st, pairs, pp, boys, p0, np);
+  hipLaunchKernelGGL((schwarz_kernel<1, 1>), dim3((np + 3) / 4), dim3(256), 0, st, pairs, pp, boys, p0, np);
   return hipGetLastError();
 }
 

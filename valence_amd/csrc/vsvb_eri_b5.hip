@@ -15,9 +15,21 @@ hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks,
   }
 }
 
+int occupancy_b5(int kc, bool jk) {
+  switch (kc) {
+    case 0: return occupancy_q<2, 2, 0, 0>(jk);
+    case 1: return occupancy_q<2, 2, 1, 0>(jk);
+    case 2: return occupancy_q<2, 2, 1, 1>(jk);
+    case 3: return occupancy_q<2, 2, 2, 0>(jk);
+    case 4: return occupancy_q<2, 2, 2, 1>(jk);
+    case 5: return occupancy_q<2, 2, 2, 2>(jk);
+    default: return 1;
+  }
+}
+
 hipError_t launch_schwarz_b5(Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np, hipStream_t st) {
   if (np <= 0) return hipSuccess;
-  hipLaunchKernelGGL((schwarz_kernel<2, 2>), dim3((np + 63) / 64), dim3(64), 0, st, pairs, pp, boys, p0, np);
+  hipLaunchKernelGGL((schwarz_kernel<2, 2>), dim3((np + 3) / 4), dim3(256), 0, st, pairs, pp, boys, p0, np);
   return hipGetLastError();
 }
 

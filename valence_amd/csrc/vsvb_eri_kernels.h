@@ -4,11 +4,16 @@
 // fixed angular-momentum class with the generated Obara-Saika / Head-Gordon-
 // Pople code, then contracts the cartesian block on the fly with the AO
 // densities of every wave function that shares the geometry; no ERI is ever
-// stored.  Work is tiled 16 bra pairs x 16 ket pairs per 256-thread
-// workgroup; a fixed grid of workgroups walks the tiles (tile = block + k*grid),
-// which keeps the per-workgroup partial sums few and the reduction
-// deterministic.  Pairs inside a class are sorted by primitive-pair count, so
-// the threads of a wavefront run primitive loops of similar length.
+// stored.  A 256-thread workgroup takes a tile of 64 bra pairs x 4 ket pairs:
+// each wavefront owns one ket pair (wave-uniform: its pair and primitive data
+// are scalar loads, and the Coulomb update of the ket block is reduced across
+// the wave before one atomic per element) and the 64 bra pairs, one per lane.
+// A grid sized to the occupancy takes tiles from an atomic counter; each
+// wave keeps its own partial sum, so the partials are few (the final sum is
+// over a fixed-size array).  Pairs
+// inside a class are sorted by primitive-pair count, then by centre distance,
+// so a wavefront's lanes run primitive loops of the same length and the
+// Schwarz-skipped quartets gather in the same wavefronts.
 //
 // Density contraction (closed or open shell; RT = Coulomb density, D_s the
 // exchange densities with weight x_s), per unique quartet of degeneracy deg:
@@ -31,9 +36,9 @@ struct QuartetArgs {
   const double* boys;
   int bra0, nbra;       // pair index range of the bra class
   int ket0, nket;       // pair index range of the ket class
-  int same;             // bra class == ket class: triangle only
-  int ntk;              // ket tiles
-  int ntiles;           // tiles of this class pair
+  int same;             // bra class == ket class: only ket <= bra
+  int ntk;              // unused
+  int ntiles;           // work units: bra tiles (64 pairs) * ket pairs
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;
   int nwfn;             // wave functions sharing this geometry
@@ -43,22 +48,14 @@ struct QuartetArgs {
   double* jacc;         // J/K mode: Coulomb accumulators [nwfn][nbf][nbf]
   double* kacc;         // J/K mode: exchange accumulators [nwfn][nspin][nbf][nbf]
   unsigned long long* counters;  // [0] quartets evaluated, [1] primitive quartets, [2] skipped
+  int* tile_ctr;        // dynamic tile counter (zero before the launch)
 };
 
-constexpr int kTile = 16;
-
-__device__ __forceinline__ void tile_of(int t, int same, int ntk, int& tb, int& tk) {
-  if (same) {  // triangular enumeration: tb >= tk
-    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((i + 1) * (i + 2) / 2 <= t) ++i;
-    while (i * (i + 1) / 2 > t) --i;
-    tb = i;
-    tk = t - i * (i + 1) / 2;
-  } else {
-    tb = t / ntk;
-    tk = t % ntk;
-  }
-}
+// A work unit is 64 bra pairs x 1 ket pair for one wavefront: the ket pair is
+// wave-uniform, so its pair and primitive data are scalar loads, and the 64 bra
+// pairs take one lane each.
+constexpr int kTileB = 64;
+constexpr int kTileK = 4;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -97,31 +94,53 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
   const size_t nn = (size_t)n * n;
   const double cfac = a.nspin == 1 ? 2.0 : 1.0;  // RT = cfac * sum_s D_s
   const double xfac = a.nspin == 1 ? 2.0 : 1.0;  // exchange weight x_s
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   constexpr int kMaxW = 8;
   double esum[kMaxW];
 #pragma unroll
   for (int w = 0; w < kMaxW; ++w) esum[w] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
 
-  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    int tb, tk;
-    tile_of(tile, a.same, a.ntk, tb, tk);
-    const int ib = tb * kTile + (threadIdx.x >> 4);
-    const int ik = tk * kTile + (threadIdx.x & 15);
-    if (!(ib < a.nbra && ik < a.nket && (!a.same || ik <= ib))) continue;
-    const Pair P = a.pairs[a.bra0 + ib];
+  // Work units (64 bra pairs x 1 ket pair) are handed to wavefronts one at a
+  // time from an atomic counter, with no barrier, so wavefronts never wait for
+  // each other; in index order units run from the heaviest bra pairs (most
+  // primitive pairs) to the lightest, so the tail is short.
+  for (;;) {
+    int u = 0;
+    if (lane == 0) u = atomicAdd(a.tile_ctr, 1);
+    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
+    if (u >= a.ntiles) break;
+    const int tb = u / a.nket;
+    const int ik = __builtin_amdgcn_readfirstlane(u - tb * a.nket);
+    if (a.same && ik > tb * kTileB + kTileB - 1) continue;  // whole wave above the diagonal
+    const int ib = tb * kTileB + lane;
     const Pair K = a.pairs[a.ket0 + ik];
-    if (P.Q * K.Q < a.tau) {
-      ++nskip;
-      continue;
+    bool act = ib < a.nbra && (!a.same || ik <= ib);
+    Pair P{};
+    if (act) {
+      P = a.pairs[a.bra0 + ib];
+      if (P.Q * K.Q < a.tau) {
+        act = false;
+        ++nskip;
+      }
     }
     double v[NAB * NCD];
-    eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz,
-                              a.boys, v);
-    ++nq;
-    nprimq += (unsigned long long)P.npp * K.npp;
-    const double deg = (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0);
-    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao, mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+    if (act) {
+      eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                K.ABz, a.boys, v);
+      ++nq;
+      nprimq += (unsigned long long)P.npp * K.npp;
+    } else {
+#pragma unroll
+      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
+    }
+    if (!__any(act)) continue;  // wave-uniform: nothing to contract
+    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
+                           : 0.0;
+    // inactive lanes read a valid (ket) block and contribute zero
+    const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
+    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
     for (int w = 0; w < a.nwfn; ++w) {
       const double* D0 = a.dens + (size_t)w * a.nspin * nn;
       double rab[NAB], rcd[NCD];
@@ -173,76 +192,81 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
         // Coulomb: A_mn += deg/4 sum v RT_ls ; A_ls += deg/4 sum v RT_mn      (J = A + A^T)
         double* J = a.jacc + (size_t)w * nn;
         const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+        if (act) {
 #pragma unroll
-        for (int ij = 0; ij < NAB; ++ij) {
-          double t = 0.0;
+          for (int ij = 0; ij < NAB; ++ij) {
+            double t = 0.0;
 #pragma unroll
-          for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-          unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
+            for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+            unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
+          }
         }
+        // ket block is shared by the wave: reduce over lanes, one atomic per element
 #pragma unroll
         for (int kl = 0; kl < NCD; ++kl) {
           double t = 0.0;
 #pragma unroll
           for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
-          unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], fj * t);
+          t = wave_sum(fj * t);
+          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], t);
         }
         // Exchange (carries x_s): B_ml += v D_ns ; B_nl += v D_ms ; B_ms += v D_nl ; B_ns += v D_ml  (K = B + B^T)
-        for (int s = 0; s < a.nspin; ++s) {
-          DBlocks<NA, NB, NC, ND> d;
-          d.load(D0 + s * nn, n, ma, mb, mc, md);
-          double* Kx = a.kacc + ((size_t)w * a.nspin + s) * nn;
+        if (act) {
+          for (int s = 0; s < a.nspin; ++s) {
+            DBlocks<NA, NB, NC, ND> d;
+            d.load(D0 + s * nn, n, ma, mb, mc, md);
+            double* Kx = a.kacc + ((size_t)w * a.nspin + s) * nn;
 #pragma unroll
-          for (int i = 0; i < NA; ++i)
+            for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
+                for (int j = 0; j < NB; ++j)
 #pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
-              unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
-            }
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
+                unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
+              }
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
 #pragma unroll
-              for (int i = 0; i < NA; ++i)
+                for (int i = 0; i < NA; ++i)
 #pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
-              unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
-            }
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
+                unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
+              }
 #pragma unroll
-          for (int i = 0; i < NA; ++i)
+            for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int l = 0; l < ND; ++l) {
-              double t = 0.0;
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
+                for (int j = 0; j < NB; ++j)
 #pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
-              unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
-            }
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
+                unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
+              }
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
-            for (int l = 0; l < ND; ++l) {
-              double t = 0.0;
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
 #pragma unroll
-              for (int i = 0; i < NA; ++i)
+                for (int i = 0; i < NA; ++i)
 #pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
-              unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
-            }
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
+                unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
+              }
+          }
         }
       }
     }
   }
   if constexpr (!kJK) {
     // deterministic: shuffle-reduce each wave, one partial per (wave, wave function)
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = a.nwfn < kMaxW ? a.nwfn : kMaxW;
     for (int w = 0; w < nw; ++w) {
       double s = 0.0;
@@ -260,7 +284,7 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
       pq += __shfl_xor(pq, o, 64);
       sk += __shfl_xor(sk, o, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
       atomicAdd(&a.counters[0], q);
       atomicAdd(&a.counters[1], pq);
       atomicAdd(&a.counters[2], sk);
@@ -268,20 +292,41 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
   }
 }
 
-// Schwarz bound of every pair of one class: Q = sqrt(max_ab |(ab|ab)|).
+// Schwarz bound of every pair of one class, Q = sqrt(max_ab |(ab|ab)|): one
+// wavefront per pair, the lanes split the bra primitive pairs and the partial
+// diagonal integrals are summed across the wave.
 template <int LA, int LB>
-__global__ void __launch_bounds__(64) schwarz_kernel(Pair* pairs, const PrimPair* pp, const double* boys, int p0,
-                                                     int np) {
+__global__ void __launch_bounds__(256) schwarz_kernel(Pair* pairs, const PrimPair* pp, const double* boys, int p0,
+                                                      int np) {
   constexpr int NAB = ncart(LA) * ncart(LB);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (i >= np) return;
   const Pair P = pairs[p0 + i];
-  double v[NAB * NAB];
-  eri_block<LA, LB, LA, LB>(pp + P.pp0, P.npp, pp + P.pp0, P.npp, P.ABx, P.ABy, P.ABz, P.ABx, P.ABy, P.ABz, boys, v);
+  double dsum[NAB];
+#pragma unroll
+  for (int k = 0; k < NAB; ++k) dsum[k] = 0.0;
+  for (int b = lane; b < P.npp; b += 64) {
+    double v[NAB * NAB];
+    eri_block<LA, LB, LA, LB>(pp + P.pp0 + b, 1, pp + P.pp0, P.npp, P.ABx, P.ABy, P.ABz, P.ABx, P.ABy, P.ABz, boys,
+                              v);
+#pragma unroll
+    for (int k = 0; k < NAB; ++k) dsum[k] += v[k * NAB + k];
+  }
   double m = 0.0;
 #pragma unroll
-  for (int k = 0; k < NAB; ++k) m = fmax(m, fabs(v[k * NAB + k]));
-  pairs[p0 + i].Q = sqrt(m);
+  for (int k = 0; k < NAB; ++k) m = fmax(m, fabs(wave_sum(dsum[k])));
+  if (lane == 0) pairs[p0 + i].Q = sqrt(m);
+}
+
+template <int LA, int LB, int LC, int LD>
+static int occupancy_q(bool jk) {
+  int n = 0;
+  if (jk)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, true>, 256, 0);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, false>, 256, 0);
+  return n > 0 ? n : 1;
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -301,6 +346,12 @@ hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, int nblocks,
 hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
 hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
 hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
+int occupancy_b0(int kc, bool jk);
+int occupancy_b1(int kc, bool jk);
+int occupancy_b2(int kc, bool jk);
+int occupancy_b3(int kc, bool jk);
+int occupancy_b4(int kc, bool jk);
+int occupancy_b5(int kc, bool jk);
 hipError_t launch_schwarz_b0(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b1(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b2(Pair*, const PrimPair*, const double*, int, int, hipStream_t);

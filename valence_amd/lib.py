@@ -46,7 +46,8 @@ class COptions(C.Structure):
 EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_energy_parts", "vsvb_energy_grad",
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
-           "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_version"]
+           "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
+           "vsvb_version"]
 
 _lib = None
 
@@ -75,6 +76,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_last_counters.argtypes = [C.c_void_p, _f64p]
     L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
     L.vsvb_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_debug_boys.argtypes = [C.c_int32, _f64p, C.c_int32, _f64p]
     L.vsvb_class_stats.argtypes = [C.c_void_p, _f64p]
     L.vsvb_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_batch_class_stats.argtypes = [C.c_void_p, _f64p]
@@ -91,6 +93,14 @@ def _check(rc: int, what: str):
     if rc != VSVB_OK:
         msg = load().vsvb_last_error().decode(errors="replace")
         raise VsvbError(f"{what} failed ({rc}): {msg}")
+
+
+def device_boys(L: int, T) -> np.ndarray:
+    """F_0..F_L at the given arguments, evaluated by the device routine the ERI kernels use."""
+    T = np.ascontiguousarray(T, np.float64).ravel()
+    out = np.zeros((len(T), L + 1))
+    _check(load().vsvb_debug_boys(L, _ptr(T, _f64p), len(T), _ptr(out, _f64p)), "vsvb_debug_boys")
+    return out
 
 
 class _Packed:
