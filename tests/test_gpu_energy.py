@@ -135,3 +135,18 @@ def test_batch_matches_single(lib):
             e2 = ctx.energy()
         assert abs(e[k] - e1) < 1e-9 and abs(e[k] - e2) < 1e-9
         assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)
+
+
+def test_batch_groups_share_integrals(lib):
+    """Wave functions on one molecule share a single ERI pass; results must equal
+    independent evaluations (rotations of C8H18 and one displaced geometry)."""
+    systems = workloads.config3()
+    pick = [48, 49, 50, 60, 0, 90]          # four rotations (one group), one displaced, one alkane
+    b = lib.Batch([systems[i] for i in pick])
+    e, g = b.run(want_grad=True)
+    for k, i in enumerate(pick):
+        with lib.Context(systems[i]) as ctx:
+            e1, g1 = ctx.energy_grad()
+        print(f"wfn {i}: batch {e[k]:.12f} single {e1:.12f}")
+        assert abs(e[k] - e1) < 1e-9
+        assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)

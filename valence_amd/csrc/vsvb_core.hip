@@ -143,6 +143,31 @@ __global__ void k_pair_geom(Pair* __restrict__ pairs, int np, const Shell* __res
   pairs[t].ABz = xyz[3 * A.atom + 2] - xyz[3 * B.atom + 2];
 }
 
+// Primitive-pair screening: keep a shell pair's primitive pairs with
+// |K_ab| > kPrimCut (K carries the contraction weights and exp(-ab/(a+b) |AB|^2)),
+// compacted to the front in their original order; the pair's npp becomes the
+// kept count.  A dropped primitive pair changes any ERI by less than
+// kPrimCut * (the largest K) * (a bounded Boys/angular factor): far below 1e-15
+// relative, orders of magnitude inside the 1e-9 Eh energy gate.
+constexpr double kPrimCut = 1e-20;
+
+__global__ void k_prim_screen(Pair* __restrict__ pairs, int np, const Shell* __restrict__ shells,
+                              PrimPair* __restrict__ pp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  Pair P = pairs[t];
+  const int cap = shells[P.A].nprim * shells[P.B].nprim;
+  int k = 0;
+  for (int i = 0; i < cap; ++i) {
+    const PrimPair q = pp[P.pp0 + i];
+    if (fabs(q.K) > kPrimCut) {
+      if (k != i) pp[P.pp0 + k] = q;
+      ++k;
+    }
+  }
+  pairs[t].npp = k;
+}
+
 // ---------------------------------------------------------------------------
 // device: one-electron integrals (Obara-Saika), one thread per shell pair
 // ---------------------------------------------------------------------------
@@ -552,12 +577,22 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool own = true;
   hipError_t alloc(size_t count) {
     n = count;
+    own = true;
     return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
   }
+  // Point at memory owned elsewhere (a batch group shares geometry data and
+  // slices of group-wide density / Coulomb / exchange arrays).
+  void alias(T* q, size_t count) {
+    if (p && own) (void)hipFree(p);
+    p = q;
+    n = count;
+    own = false;
+  }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
   }
 };
 
@@ -599,6 +634,7 @@ struct vsvb_context {
   // per-class profiling (events around each class launch when enabled)
   bool profile = false;
   bool last_jk = false;
+  bool ran_quartets = false;  // this context launched the quartet stage in the current call
   hipEvent_t cls_ev[kNQuartetClass + 1] = {};
   double class_stats[kNQuartetClass][6] = {};
 };
@@ -890,6 +926,9 @@ static int prepare_geometry(vsvb_context* c, const double* coords) {
   hipLaunchKernelGGL(k_pair_prep, dim3((c->npp + 255) / 256), dim3(256), 0, st, c->d_ppsrc.p, c->npp, c->d_pairs.p,
                      c->d_shells.p, c->d_pexp.p, c->d_pcoef.p, c->d_xyz.p, kfold, c->d_pp.p);
   HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_prim_screen, dim3((c->npairs + 255) / 256), dim3(256), 0, st, c->d_pairs.p, c->npairs,
+                     c->d_shells.p, c->d_pp.p);
+  HIP_OK(hipGetLastError());
   for (int pc = 0; pc < kNPairClass; ++pc) {
     const int p0 = c->pair_off[pc], np = c->pair_off[pc + 1] - p0;
     HIP_OK(launch_one_e(pc, c->d_pairs.p, p0, np, c->d_shells.p, c->d_pexp.p, c->d_pcoef.p, c->d_xyz.p,
@@ -957,16 +996,23 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
   return VSVB_OK;
 }
 
-static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
+// Shell-quartet stage.  By default it serves this context's own wave function;
+// a batch group leader passes nwfn > 1 and group-wide density and J/K arrays.
+static int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, const double* dens = nullptr,
+                        double* jacc = nullptr, double* kacc = nullptr) {
   hipStream_t st = c->stream;
   const int nspin = c->nunpd > 0 ? 2 : 1;
   const size_t nn = (size_t)c->nbf * c->nbf;
+  if (!dens) dens = c->d_R.p;
+  if (!jacc) jacc = c->d_J.p;
+  if (!kacc) kacc = c->d_K.p;
   HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * kNQuartetClass * sizeof(unsigned long long), st));
   HIP_OK(hipMemsetAsync(c->d_tilectr.p, 0, kNQuartetClass * sizeof(int), st));
   c->last_jk = jk;
+  c->ran_quartets = true;
   if (jk) {
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_J.p, nn, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_K.p, nspin * nn, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, kacc, nwfn * nspin * nn, 0.0);
   }
   HIP_OK(hipEventRecord(c->ev0, st));
   size_t poff = 0;
@@ -987,12 +1033,12 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev) {
       a.ntiles = c->ntiles_class[q];
       a.tau = c->tau;
       a.nbf = c->nbf;
-      a.nwfn = 1;
+      a.nwfn = nwfn;
       a.nspin = nspin;
-      a.dens = c->d_R.p;
+      a.dens = dens;
       a.partial = c->d_partial.p + poff;
-      a.jacc = c->d_J.p;
-      a.kacc = c->d_K.p;
+      a.jacc = jacc;
+      a.kacc = kacc;
       a.counters = c->d_counters.p + 4 * q;
       a.tile_ctr = c->d_tilectr.p + q;
       if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
@@ -1020,6 +1066,12 @@ static double digestion_flops(int q, int nspin, bool jk) {
 }
 
 static int finish_counters(vsvb_context* c) {
+  if (!c->ran_quartets) {  // a batch-group member: its integrals were counted by the leader
+    std::memset(c->class_stats, 0, sizeof(c->class_stats));
+    std::memset(c->last_counters, 0, sizeof(c->last_counters));
+    return VSVB_OK;
+  }
+  c->ran_quartets = false;
   unsigned long long h[4 * kNQuartetClass];
   HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0.0f;
@@ -1158,14 +1210,56 @@ static int finish_energy_grad(vsvb_context* c, double* energy, double* grad) {
 // First batched mode: one context per wave function, every evaluation enqueued
 // on its own stream before any host synchronization, so independent wave
 // functions overlap on the device.
+// Wave functions on the same molecule (geometry, atom types and basis) and of
+// the same spin type form a group: the group leader owns the geometry data and
+// runs one shell-quartet pass whose thread blocks contract each ERI block with
+// every member's density; members alias the leader's geometry buffers and use
+// slices of the group's density / Coulomb / exchange arrays.
+struct BatchGroup {
+  std::vector<int> members;  // indices into vsvb_batch::ctx; members[0] leads
+  DevBuf<double> dens, jacc, kacc;
+};
+
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
+  std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;
   ~vsvb_batch() {
+    groups.clear();
     for (auto* c : ctx) vsvb_context_destroy(c);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+static bool same_molecule(const vsvb_system* a, const vsvb_system* b) {
+  if (a->natom != b->natom || a->ntype != b->ntype || a->nshell_type != b->nshell_type || a->nprim != b->nprim)
+    return false;
+  if ((a->nunpd > 0) != (b->nunpd > 0) || a->nspinc != b->nspinc || a->npair != b->npair) return false;
+  auto eq = [](const void* x, const void* y, size_t n) { return std::memcmp(x, y, n) == 0; };
+  return eq(a->atom_type, b->atom_type, sizeof(int32_t) * a->natom) &&
+         eq(a->coords, b->coords, sizeof(double) * 3 * a->natom) &&
+         eq(a->nuc_charge, b->nuc_charge, sizeof(double) * a->ntype) &&
+         eq(a->type_shell_start, b->type_shell_start, sizeof(int32_t) * (a->ntype + 1)) &&
+         eq(a->shell_l, b->shell_l, sizeof(int32_t) * a->nshell_type) &&
+         eq(a->shell_prim_start, b->shell_prim_start, sizeof(int32_t) * (a->nshell_type + 1)) &&
+         eq(a->prim_exp, b->prim_exp, sizeof(double) * a->nprim) && eq(a->prim_coef, b->prim_coef, sizeof(double) * a->nprim);
+}
+
+// Make member m of a group share the leader's geometry data.
+static void share_geometry(vsvb_context* lead, vsvb_context* m) {
+  m->d_shells.alias(lead->d_shells.p, lead->d_shells.n);
+  m->d_pairs.alias(lead->d_pairs.p, lead->d_pairs.n);
+  m->d_ppsrc.alias(lead->d_ppsrc.p, lead->d_ppsrc.n);
+  m->d_pp.alias(lead->d_pp.p, lead->d_pp.n);
+  m->d_pexp.alias(lead->d_pexp.p, lead->d_pexp.n);
+  m->d_pcoef.alias(lead->d_pcoef.p, lead->d_pcoef.n);
+  m->d_xyz.alias(lead->d_xyz.p, lead->d_xyz.n);
+  m->d_charge.alias(lead->d_charge.p, lead->d_charge.n);
+  m->d_boys.alias(lead->d_boys.p, lead->d_boys.n);
+  m->d_S.alias(lead->d_S.p, lead->d_S.n);
+  m->d_H.alias(lead->d_H.p, lead->d_H.n);
+  m->geom_ready = true;  // members never rebuild geometry; the leader does
+}
 
 // Device Boys function at given arguments (diagnostics / tests).
 template <int L>
@@ -1336,27 +1430,70 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
     }
     bt->ctx.push_back(c);
   }
+  // group wave functions that share the molecule
+  std::vector<int> gid(nwfn, -1);
+  for (int i = 0; i < nwfn; ++i) {
+    if (gid[i] >= 0) continue;
+    gid[i] = (int)bt->groups.size();
+    bt->groups.emplace_back();
+    bt->groups.back().members.push_back(i);
+    for (int j = i + 1; j < nwfn; ++j)
+      if (gid[j] < 0 && same_molecule(systems[i], systems[j])) {
+        gid[j] = gid[i];
+        bt->groups.back().members.push_back(j);
+      }
+  }
+  for (auto& g : bt->groups) {
+    vsvb_context* lead = bt->ctx[g.members[0]];
+    if (g.members.size() < 2) continue;
+    const size_t nn = (size_t)lead->nbf * lead->nbf;
+    const int nspin = lead->nunpd > 0 ? 2 : 1;
+    const size_t G = g.members.size();
+    HIP_OK(g.dens.alloc(G * nspin * nn));
+    HIP_OK(g.jacc.alloc(G * nn));
+    HIP_OK(g.kacc.alloc(G * nspin * nn));
+    for (size_t k = 0; k < G; ++k) {
+      vsvb_context* m = bt->ctx[g.members[k]];
+      if (k > 0) share_geometry(lead, m);
+      m->d_R.alias(g.dens.p + k * nspin * nn, nspin * nn);
+      m->d_J.alias(g.jacc.p + k * nn, nn);
+      m->d_K.alias(g.kacc.p + k * nspin * nn, nspin * nn);
+    }
+  }
   *b = bt;
   return VSVB_OK;
 }
 
 int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* const* grads) {
   if (!b) return fail(VSVB_ERR_ARG, "null batch");
-  const size_t n = b->ctx.size();
-  for (size_t i = 0; i < n; ++i) {
-    vsvb_context* c = b->ctx[i];
-    HIP_OK(hipSetDevice(c->device));
-    // geometry-dependent data (pair data, one-electron integrals, Schwarz bounds)
-    // are rebuilt every run: passing the coordinates forces it.
-    const int rc = want_grad ? enqueue_energy_grad(c, c->coords.data(), nullptr, nullptr, c->d_grad.p)
-                             : enqueue_energy(c, c->coords.data(), nullptr);
-    if (rc) return rc;
+  int rc;
+  // Every wave function goes through the J/K (adjoint-ready) contraction; the
+  // energy is assembled from it, and the gradient when asked for.
+  for (auto& g : b->groups) {
+    vsvb_context* lead = b->ctx[g.members[0]];
+    HIP_OK(hipSetDevice(lead->device));
+    for (int i : g.members)
+      for (int k = 0; k < lead->totlen && i == g.members[0]; ++k)
+        if (lead->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
+    // geometry-dependent data are rebuilt every run: passing the coordinates forces it
+    if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
+    for (int i : g.members)
+      if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
+    if (g.members.size() > 1) {
+      if ((rc = run_quartets(lead, true, nullptr, (int)g.members.size(), g.dens.p, g.jacc.p, g.kacc.p))) return rc;
+    } else if ((rc = run_quartets(lead, true, nullptr))) {
+      return rc;
+    }
+    for (int i : g.members) {
+      vsvb_context* m = b->ctx[i];
+      m->enuc = lead->enuc;
+      if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
+    }
   }
-  for (size_t i = 0; i < n; ++i) {
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
     vsvb_context* c = b->ctx[i];
-    const int rc = want_grad ? finish_energy_grad(c, energies ? energies + i : nullptr, grads ? grads[i] : nullptr)
-                             : finish_energy(c, energies ? energies + i : nullptr, nullptr);
-    if (rc) return rc;
+    if ((rc = finish_energy_grad(c, energies ? energies + i : nullptr, (want_grad && grads) ? grads[i] : nullptr)))
+      return rc;
   }
   return VSVB_OK;
 }
