@@ -19,9 +19,9 @@ LIB = os.path.join(HERE, "libvsvb_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["vsvb_core.hip"] + [f"vsvb_eri_b{b}.hip" for b in range(6)]
+SOURCES = ["vsvb_core.hip", "vsvb_sc.hip"] + [f"vsvb_eri_b{b}.hip" for b in range(6)]
 ERI_HEADERS = ["vsvb_device.h", "vsvb_eri_kernels.h", "eri_gen.inc"]
-CORE_HEADERS = ERI_HEADERS + ["eri_flops.inc", os.path.join("..", "..", "include", "vsvb_hip.h")]
+HOST_HEADERS = ["vsvb_internal.h", "eri_flops.inc", os.path.join("..", "..", "include", "vsvb_hip.h")]
 
 # Packed FP32 instructions are disabled for all device code (they have given
 # silent wrong results on this hardware pool); the host pass ignores the flag.
@@ -32,7 +32,7 @@ CFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno
 
 def _digest(src: str) -> str:
     h = hashlib.sha256()
-    for f in [src] + (CORE_HEADERS if src == "vsvb_core.hip" else ERI_HEADERS):
+    for f in [src] + (ERI_HEADERS + HOST_HEADERS if not src.startswith("vsvb_eri_b") else ERI_HEADERS):
         p = os.path.join(CSRC, f)
         if os.path.exists(p):
             with open(p, "rb") as fh:

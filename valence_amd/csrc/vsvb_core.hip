@@ -21,8 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/vsvb_hip.h"
-#include "vsvb_eri_kernels.h"
+#include "vsvb_internal.h"
 
 namespace vsvb {
 #include "eri_flops.inc"
@@ -33,19 +32,14 @@ using namespace vsvb;
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------
-static thread_local std::string g_last_error;
+namespace vsvb {
+thread_local std::string g_last_error;
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
-
-#define HIP_OK(expr)                                                                                   \
-  do {                                                                                                 \
-    hipError_t _e = (expr);                                                                            \
-    if (_e != hipSuccess)                                                                              \
-      return fail(VSVB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));                 \
-  } while (0)
+}  // namespace vsvb
 
 // ---------------------------------------------------------------------------
 // host helpers: normalization (norm_prim, valence.F90:2264-2306), angn
@@ -102,9 +96,6 @@ static std::vector<double> make_boys_table() {
 // ---------------------------------------------------------------------------
 // device: geometry preparation
 // ---------------------------------------------------------------------------
-struct PPSrc {
-  int pair, ia, ib, pad;  // pair index, global primitive indices of A and B
-};
 
 __global__ void k_pair_prep(const PPSrc* __restrict__ src, int npp, const Pair* __restrict__ pairs,
                             const Shell* __restrict__ shells, const double* __restrict__ pexp,
@@ -362,12 +353,6 @@ static hipError_t launch_one_e(int pc, const Pair* pairs, int p0, int np, const 
 // ---------------------------------------------------------------------------
 // device: orbital weights
 // ---------------------------------------------------------------------------
-struct OrbEntry {
-  int orb;   // owning orbital
-  int kind;  // 0: AO, 1: NDF reference
-  int tgt;   // AO index (kind 0) or NDF orbital index (kind 1)
-  int pad;
-};
 
 // NDF self-overlap scale n_f^-1/2 (normal over the NDF range, valence.F90:144).
 __global__ void k_ndf_scale(const OrbEntry* __restrict__ ent, const int* __restrict__ orb_start,
@@ -465,8 +450,8 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, double alpha,
   }
 }
 
-static hipError_t gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
-                       const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
+hipError_t vsvb::gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
+                      const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 g((N + 15) / 16, (M + 15) / 16);
   hipLaunchKernelGGL(k_gemm, g, dim3(256), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
@@ -545,6 +530,17 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+hipError_t vsvb::launch_fill(hipStream_t st, double* p, size_t n, double v) {
+  const unsigned nb = (unsigned)std::min<size_t>(1024, (n + 255) / 256 + 1);
+  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t vsvb::launch_reduce(hipStream_t st, const double* partial, long n, int nw, double* out) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, n, nw, out);
+  return hipGetLastError();
+}
+
 // M <- a*(X + X^T) + b*Y (elementwise, n x n); X, Y may alias nothing of M.
 __global__ void k_sym_combine(const double* __restrict__ X, double a, const double* __restrict__ Y, double b, int n,
                               double* __restrict__ M) {
@@ -565,6 +561,12 @@ __global__ void k_gather_grad(const OrbEntry* __restrict__ ent, int totlen, int 
   grad[k] = (e.kind == 0 && e.orb < nsp) ? angn[e.tgt] * G[(size_t)e.orb * nbf + e.tgt] : 0.0;
 }
 
+hipError_t vsvb::launch_gather_grad(vsvb_context* c, const double* gradC, double* grad) {
+  hipLaunchKernelGGL(k_gather_grad, dim3((c->totlen + 255) / 256), dim3(256), 0, c->stream, c->d_ent.p, c->totlen,
+                     c->nsp, c->d_angn.p, gradC, c->nbf, grad);
+  return hipGetLastError();
+}
+
 __global__ void k_final_energy(const double* __restrict__ e1, const double* __restrict__ e2, double enuc,
                                double* __restrict__ out) {
   out[0] = e1[0] + e2[0] + enuc;
@@ -573,71 +575,6 @@ __global__ void k_final_energy(const double* __restrict__ e1, const double* __re
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  bool own = true;
-  hipError_t alloc(size_t count) {
-    n = count;
-    own = true;
-    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
-  }
-  // Point at memory owned elsewhere (a batch group shares geometry data and
-  // slices of group-wide density / Coulomb / exchange arrays).
-  void alias(T* q, size_t count) {
-    if (p && own) (void)hipFree(p);
-    p = q;
-    n = count;
-    own = false;
-  }
-  ~DevBuf() {
-    if (p && own) (void)hipFree(p);
-  }
-};
-
-struct vsvb_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = true;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double tau = 1e-17;
-  // sizes
-  int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
-  int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;
-  double dtol = 0.0;
-  int pair_off[kNPairClass + 1] = {0};
-  // host copies
-  std::vector<double> coords, charges, coeff;
-  std::vector<OrbEntry> ent;
-  std::vector<int> orb_start;
-  // device
-  DevBuf<Shell> d_shells;
-  DevBuf<Pair> d_pairs;
-  DevBuf<PPSrc> d_ppsrc;
-  DevBuf<PrimPair> d_pp;
-  DevBuf<double> d_pexp, d_pcoef, d_xyz, d_charge, d_boys, d_angn;
-  DevBuf<int> d_ao_shell, d_orb_start, d_status;
-  DevBuf<OrbEntry> d_ent;
-  DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
-  DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
-  DevBuf<unsigned long long> d_counters;
-  DevBuf<int> d_tilectr;
-  int blocks_jk[kNQuartetClass] = {0};
-  size_t partial_per_class[kNQuartetClass] = {0};
-  int blocks_per_class[kNQuartetClass] = {0};
-  int ntiles_class[kNQuartetClass] = {0};
-  size_t partial_total = 0;
-  bool geom_ready = false;
-  double enuc = 0.0;
-  double last_counters[4] = {0, 0, 0, 0};
-  // per-class profiling (events around each class launch when enabled)
-  bool profile = false;
-  bool last_jk = false;
-  bool ran_quartets = false;  // this context launched the quartet stage in the current call
-  hipEvent_t cls_ev[kNQuartetClass + 1] = {};
-  double class_stats[kNQuartetClass][6] = {};
-};
 
 static double nuclear_repulsion(int natom, const double* xyz, const double* Z) {
   // get_nuclear_repulsion_energy (src/tools_module.F90:19-40)
@@ -703,7 +640,6 @@ static int quartet_class_index(int bc, int kc) { return bc * (bc + 1) / 2 + kc; 
 
 static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_context* c) {
   if (!s) return fail(VSVB_ERR_ARG, "null system");
-  if (s->npair > 0) return fail(VSVB_ERR_UNSUPPORTED, "spin-coupled pairs are not handled by this context type");
   if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
   c->device = opt ? opt->device : 0;
   c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
@@ -909,11 +845,12 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(hipMemcpy(c->d_orb_start.p, c->orb_start.data(), c->orb_start.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_ent.p, c->ent.data(), c->ent.size() * sizeof(OrbEntry), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_coeff.p, c->coeff.data(), c->totlen * sizeof(double), hipMemcpyHostToDevice));
+  if (s->npair > 0) return sc_setup(c, s);
   return VSVB_OK;
 }
 
 // geometry-dependent device state
-static int prepare_geometry(vsvb_context* c, const double* coords) {
+int vsvb::prepare_geometry(vsvb_context* c, const double* coords) {
   if (coords) c->coords.assign(coords, coords + 3 * c->natom);
   if (c->geom_ready && !coords) return VSVB_OK;
   hipStream_t st = c->stream;
@@ -942,7 +879,7 @@ static int prepare_geometry(vsvb_context* c, const double* coords) {
 // weights -> Ct, Cs, orbital S and h, P per spin block, R per spin block, E1.
 // Spin blocks (npair = 0): alpha = all spatial orbitals (unpaired then docc),
 // beta = the trailing ndocc (docc); closed shell when nunpd = 0.
-static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const double* coeff_dev) {
+int vsvb::prepare_orbital_basis(vsvb_context* c, const double* coeff_host, const double* coeff_dev) {
   hipStream_t st = c->stream;
   if (coeff_host) {
     c->coeff.assign(coeff_host, coeff_host + c->totlen);
@@ -973,6 +910,14 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
                        ns, c->d_shells.p, c->nsh, c->d_angn.p, n, c->dtol, c->d_Cs.p, c->d_status.p + 1);
     HIP_OK(hipGetLastError());
   }
+  return VSVB_OK;
+}
+
+static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const double* coeff_dev) {
+  int rc;
+  if ((rc = prepare_orbital_basis(c, coeff_host, coeff_dev))) return rc;
+  hipStream_t st = c->stream;
+  const int n = c->nbf, ns = c->nsp;
   const int nspin = c->nunpd > 0 ? 2 : 1;
   for (int s = 0; s < nspin; ++s) {
     const int off = s == 0 ? 0 : c->nunpd;
@@ -998,10 +943,12 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
 
 // Shell-quartet stage.  By default it serves this context's own wave function;
 // a batch group leader passes nwfn > 1 and group-wide density and J/K arrays.
-static int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, const double* dens = nullptr,
-                        double* jacc = nullptr, double* kacc = nullptr) {
+// nspin_override = 0 turns the J/K launch into a Coulomb-only pass over nwfn
+// general symmetric densities (the exchange accumulators are not touched).
+int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const double* dens, double* jacc,
+                       double* kacc, int nspin_override) {
   hipStream_t st = c->stream;
-  const int nspin = c->nunpd > 0 ? 2 : 1;
+  const int nspin = nspin_override >= 0 ? nspin_override : (c->nunpd > 0 ? 2 : 1);
   const size_t nn = (size_t)c->nbf * c->nbf;
   if (!dens) dens = c->d_R.p;
   if (!jacc) jacc = c->d_J.p;
@@ -1009,6 +956,8 @@ static int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, 
   HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * kNQuartetClass * sizeof(unsigned long long), st));
   HIP_OK(hipMemsetAsync(c->d_tilectr.p, 0, kNQuartetClass * sizeof(int), st));
   c->last_jk = jk;
+  c->last_nwfn = nwfn;
+  c->last_nspin = nspin;
   c->ran_quartets = true;
   if (jk) {
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
@@ -1065,7 +1014,7 @@ static double digestion_flops(int q, int nspin, bool jk) {
   return jk ? nab * ncd * (4.0 + 8.0 * nspin) : nab * ncd * (2.0 + 4.0 * nspin) + 2.0 * nab;
 }
 
-static int finish_counters(vsvb_context* c) {
+int vsvb::finish_counters(vsvb_context* c) {
   if (!c->ran_quartets) {  // a batch-group member: its integrals were counted by the leader
     std::memset(c->class_stats, 0, sizeof(c->class_stats));
     std::memset(c->last_counters, 0, sizeof(c->last_counters));
@@ -1076,7 +1025,7 @@ static int finish_counters(vsvb_context* c) {
   HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  const int nspin = c->nunpd > 0 ? 2 : 1;
+  const int nspin = c->last_nspin;
   double tot[3] = {0, 0, 0};
   for (int q = 0; q < kNQuartetClass; ++q) {
     for (int k = 0; k < 3; ++k) tot[k] += (double)h[4 * q + k];
@@ -1087,7 +1036,7 @@ static int finish_counters(vsvb_context* c) {
     c->class_stats[q][2] = (double)h[4 * q + 1];
     c->class_stats[q][3] = (double)h[4 * q + 2];
     c->class_stats[q][4] = (double)h[4 * q + 1] * kFlopsPerPrimQuartet[q] +
-                           (double)h[4 * q] * (kFlopsHrr[q] + digestion_flops(q, nspin, c->last_jk));
+                           (double)h[4 * q] * (kFlopsHrr[q] + c->last_nwfn * digestion_flops(q, nspin, c->last_jk));
     c->class_stats[q][5] = c->ntiles_class[q] > 0 ? 1.0 : 0.0;
   }
   c->last_counters[0] = tot[0];
@@ -1097,11 +1046,14 @@ static int finish_counters(vsvb_context* c) {
   return VSVB_OK;
 }
 
-static int check_status(vsvb_context* c, bool for_grad) {
+int vsvb::check_status(vsvb_context* c, bool for_grad) {
   int st[4];
   HIP_OK(hipMemcpyAsync(st, c->d_status.p, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   if (st[0]) return fail(VSVB_ERR_NUMERIC, "orbital overlap block is not positive definite (linearly dependent orbitals)");
+  if (st[2])
+    return fail(VSVB_ERR_NUMERIC, "a transition overlap between two determinants of the spin-coupled expansion is "
+                                  "exactly singular (cofactors of singular blocks are not implemented)");
   if (for_grad && st[1])
     return fail(VSVB_ERR_UNSUPPORTED, "gradient requested while the per-shell weight screen drops shells (raise ntol_d)");
   return VSVB_OK;
@@ -1139,6 +1091,12 @@ static int finish_energy(vsvb_context* c, double* energy, double* parts) {
 static int energy_impl(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* parts) {
   int rc;
   HIP_OK(hipSetDevice(c->device));
+  if (c->npair > 0) {
+    if ((rc = prepare_geometry(c, coords))) return rc;
+    if ((rc = prepare_orbital_basis(c, coeff, nullptr))) return rc;
+    if ((rc = sc_enqueue(c, false, nullptr))) return rc;
+    return sc_finish(c, energy, parts, nullptr);
+  }
   if ((rc = enqueue_energy(c, coords, coeff))) return rc;
   return finish_energy(c, energy, parts);
 }
@@ -1235,6 +1193,7 @@ static bool same_molecule(const vsvb_system* a, const vsvb_system* b) {
   if (a->natom != b->natom || a->ntype != b->ntype || a->nshell_type != b->nshell_type || a->nprim != b->nprim)
     return false;
   if ((a->nunpd > 0) != (b->nunpd > 0) || a->nspinc != b->nspinc || a->npair != b->npair) return false;
+  if (a->npair > 0) return false;  // spin-coupled wave functions run their own pass
   auto eq = [](const void* x, const void* y, size_t n) { return std::memcmp(x, y, n) == 0; };
   return eq(a->atom_type, b->atom_type, sizeof(int32_t) * a->natom) &&
          eq(a->coords, b->coords, sizeof(double) * 3 * a->natom) &&
@@ -1344,7 +1303,16 @@ int vsvb_energy_parts(vsvb_context* c, const double* coeff, double* num, double*
 int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff, double* energy, double* grad) {
   if (!c) return fail(VSVB_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
-  int rc = enqueue_energy_grad(c, coords, coeff, nullptr, c->d_grad.p);
+  int rc;
+  if (c->npair > 0) {
+    for (int i = 0; i < c->totlen; ++i)
+      if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
+    if ((rc = prepare_geometry(c, coords))) return rc;
+    if ((rc = prepare_orbital_basis(c, coeff, nullptr))) return rc;
+    if ((rc = sc_enqueue(c, true, c->d_grad.p))) return rc;
+    return sc_finish(c, energy, nullptr, grad);
+  }
+  rc = enqueue_energy_grad(c, coords, coeff, nullptr, c->d_grad.p);
   if (rc) return rc;
   return finish_energy_grad(c, energy, grad);
 }
@@ -1477,6 +1445,11 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
         if (lead->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
     // geometry-dependent data are rebuilt every run: passing the coordinates forces it
     if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
+    if (lead->npair > 0) {
+      if ((rc = prepare_orbital_basis(lead, nullptr, nullptr))) return rc;
+      if ((rc = sc_enqueue(lead, true, lead->d_grad.p))) return rc;
+      continue;
+    }
     for (int i : g.members)
       if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
     if (g.members.size() > 1) {
@@ -1492,8 +1465,9 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   }
   for (size_t i = 0; i < b->ctx.size(); ++i) {
     vsvb_context* c = b->ctx[i];
-    if ((rc = finish_energy_grad(c, energies ? energies + i : nullptr, (want_grad && grads) ? grads[i] : nullptr)))
-      return rc;
+    double* e = energies ? energies + i : nullptr;
+    double* g = (want_grad && grads) ? grads[i] : nullptr;
+    if ((rc = c->npair > 0 ? sc_finish(c, e, nullptr, g) : finish_energy_grad(c, e, g))) return rc;
   }
   return VSVB_OK;
 }

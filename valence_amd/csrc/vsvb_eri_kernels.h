@@ -42,7 +42,8 @@ struct QuartetArgs {
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;
   int nwfn;             // wave functions sharing this geometry
-  int nspin;            // 1 = closed shell (D = R, RT = 2R, x = 2); 2 = open (alpha, beta)
+  int nspin;            // 1 = closed shell (D = R, RT = 2R, x = 2); 2 = open (alpha, beta);
+                        // 0 = Coulomb only (J/K mode): J of each of nwfn general symmetric densities
   const double* dens;   // [nwfn][nspin][nbf][nbf]
   double* partial;      // energy: [gridDim.x * 4][nwfn] per-wave partial sums
   double* jacc;         // J/K mode: Coulomb accumulators [nwfn][nbf][nbf]
@@ -142,7 +143,7 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
     const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
     const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
     for (int w = 0; w < a.nwfn; ++w) {
-      const double* D0 = a.dens + (size_t)w * a.nspin * nn;
+      const double* D0 = a.dens + (size_t)w * (a.nspin > 0 ? a.nspin : 1) * nn;
       double rab[NAB], rcd[NCD];
 #pragma unroll
       for (int i = 0; i < NA; ++i)

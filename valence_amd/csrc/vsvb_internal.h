@@ -1,0 +1,161 @@
+// Internal declarations shared by the translation units of libvsvb_hip.so:
+// error reporting, device buffers, the per-molecule context and the host
+// routines of the energy pipeline that more than one unit calls.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vsvb_hip.h"
+#include "vsvb_eri_kernels.h"
+
+namespace vsvb {
+
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+
+#define HIP_OK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      return ::vsvb::fail(VSVB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));         \
+  } while (0)
+
+struct PPSrc {
+  int pair, ia, ib, pad;  // pair index, global primitive indices of A and B
+};
+
+struct OrbEntry {
+  int orb;   // owning orbital
+  int kind;  // 0: AO, 1: NDF reference
+  int tgt;   // AO index (kind 0) or NDF orbital index (kind 1)
+  int pad;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  bool own = true;
+  hipError_t alloc(size_t count) {
+    n = count;
+    own = true;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  // Point at memory owned elsewhere (a batch group shares geometry data and
+  // slices of group-wide density / Coulomb / exchange arrays).
+  void alias(T* q, size_t count) {
+    if (p && own) (void)hipFree(p);
+    p = q;
+    n = count;
+    own = false;
+  }
+  ~DevBuf() {
+    if (p && own) (void)hipFree(p);
+  }
+};
+
+
+// Spin-coupled determinant expansion (density_sc / dbra / dket,
+// src/valence.F90:1612-1870): determinant d = isc * 2^npair + mask has the
+// alpha list (pair slots, swapped where mask has the bit; unpaired; docc) and
+// the beta list (the partners; docc), as spatial-orbital indices, and weight
+// coeff_sc[isc].  Work arrays are per ordered determinant pair.
+struct SpinCoupled {
+  int ndet = 0, na = 0, nb = 0, npq = 0;
+  long npairs = 0;
+  DevBuf<int> d_alist, d_blist;          // [ndet][na], [ndet][nb]
+  DevBuf<int> d_pq;                      // [npq][2] unique orbital pairs (r >= s)
+  DevBuf<double> d_cdet;                 // [ndet]
+  DevBuf<double> d_pdens, d_pj;          // [npq][nbf][nbf] pair densities / their Coulomb matrices
+  DevBuf<double> d_W;                    // [npq][nbf][nsp]  J^{rs} C^T
+  DevBuf<double> d_eri, d_erik;          // [nsp^2][nsp^2]   (pq|rs) and its exchange ordering
+  DevBuf<double> d_Ga, d_Gb;             // [npairs][na^2], [npairs][nb^2] inverses
+  DevBuf<double> d_Xa, d_Xb, d_Xt;       // [npairs][nsp^2] transition densities
+  DevBuf<double> d_JX, d_KXa, d_KXb;     // [npairs][nsp^2]
+  DevBuf<double> d_w, d_num, d_phi, d_e1w, d_wv;  // [npairs] omega, omega*Phi, Phi, omega*e1, omega/N
+  DevBuf<double> d_Shat, d_hhat, d_hsym;          // [nsp^2] dE/dS_orb, dE/dh_orb, symmetrized copy
+  DevBuf<double> d_D2, d_Dk;                      // [nsp^2][nsp^2] dE/d(pq|rs), exchange outer product
+  DevBuf<double> d_Dh, d_Xw;                      // [nsp^2][npq], [npairs][nsp^2] adjoint work
+};
+
+}  // namespace vsvb
+
+struct vsvb_context {
+  using SpinCoupled = vsvb::SpinCoupled;
+  template <typename T> using DevBuf = vsvb::DevBuf<T>;
+  using Shell = vsvb::Shell;
+  using Pair = vsvb::Pair;
+  using PrimPair = vsvb::PrimPair;
+  using PPSrc = vsvb::PPSrc;
+  using OrbEntry = vsvb::OrbEntry;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double tau = 1e-17;
+  // sizes
+  int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
+  int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;
+  double dtol = 0.0;
+  int pair_off[vsvb::kNPairClass + 1] = {0};
+  // host copies
+  std::vector<double> coords, charges, coeff;
+  std::vector<OrbEntry> ent;
+  std::vector<int> orb_start;
+  // device
+  DevBuf<Shell> d_shells;
+  DevBuf<Pair> d_pairs;
+  DevBuf<PPSrc> d_ppsrc;
+  DevBuf<PrimPair> d_pp;
+  DevBuf<double> d_pexp, d_pcoef, d_xyz, d_charge, d_boys, d_angn;
+  DevBuf<int> d_ao_shell, d_orb_start, d_status;
+  DevBuf<OrbEntry> d_ent;
+  DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
+  DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
+  DevBuf<unsigned long long> d_counters;
+  DevBuf<int> d_tilectr;
+  int blocks_jk[vsvb::kNQuartetClass] = {0};
+  size_t partial_per_class[vsvb::kNQuartetClass] = {0};
+  int blocks_per_class[vsvb::kNQuartetClass] = {0};
+  int ntiles_class[vsvb::kNQuartetClass] = {0};
+  size_t partial_total = 0;
+  bool geom_ready = false;
+  double enuc = 0.0;
+  double last_counters[4] = {0, 0, 0, 0};
+  // per-class profiling (events around each class launch when enabled)
+  bool profile = false;
+  bool last_jk = false;
+  int last_nwfn = 1, last_nspin = 1;  // densities and spin blocks of the last quartet launch
+  bool ran_quartets = false;  // this context launched the quartet stage in the current call
+  hipEvent_t cls_ev[vsvb::kNQuartetClass + 1] = {};
+  double class_stats[vsvb::kNQuartetClass][6] = {};
+  // spin-coupled wave functions (npair > 0): determinant lists and work arrays
+  SpinCoupled sc;
+};
+
+namespace vsvb {
+
+// host routines of the pipeline (vsvb_core.hip)
+hipError_t gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
+                const double* B, int ldb, bool tb, double beta, double* C, int ldc);
+hipError_t launch_fill(hipStream_t st, double* p, size_t n, double v);
+hipError_t launch_reduce(hipStream_t st, const double* partial, long n, int nw, double* out);
+hipError_t launch_gather_grad(vsvb_context* c, const double* gradC, double* grad);
+int prepare_geometry(vsvb_context* c, const double* coords);
+int prepare_orbital_basis(vsvb_context* c, const double* coeff_host, const double* coeff_dev);
+int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, const double* dens = nullptr,
+                 double* jacc = nullptr, double* kacc = nullptr, int nspin_override = -1);
+int finish_counters(vsvb_context* c);
+int check_status(vsvb_context* c, bool for_grad);
+
+// spin-coupled path (vsvb_sc.hip)
+int sc_setup(vsvb_context* c, const vsvb_system* s);
+int sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev);
+int sc_finish(vsvb_context* c, double* energy, double* parts, double* grad_host);
+
+}  // namespace vsvb

@@ -1,0 +1,597 @@
+// Spin-coupled VSVB energy and adjoint gradient (npair > 0) on MI355X.
+//
+// The reference sums, for every orbital quartet, second-order cofactors of
+// the spin-orbital overlap over all pairs of determinants of the spin-coupled
+// expansion (density_sc, dbra, dket, det: src/valence.F90:1612-2056).  Here
+// the same sum is regrouped by determinant pair (S, T):
+//
+//   A_ST[r][c] = <alpha_S(r)|alpha_T(c)>,  B_ST likewise for beta, as
+//   build_abket orders them (src/valence.F90:2524-2586);
+//   G_a = A^-1, G_b = B^-1, omega = c_S c_T det A det B;
+//   Gamma^s[p][q] = G_s[c][r] at p = orbital of bra row r, q = of ket column c
+//   (Jacobi: a first-order cofactor is det * an inverse element, a second-order
+//   one the 2x2 determinant of inverse elements);
+//   Phi = sum h Gamma^t + 1/2 sum (pq|rs) Gamma^t_pq Gamma^t_rs
+//         - 1/2 sum_s sum (pq|rs) Gamma^s_ps Gamma^s_rq;
+//   E = sum omega Phi / sum omega + E_nuc.
+//
+// The orbital integrals (pq|rs) come from one Coulomb-only pass of the
+// shell-quartet kernels over the n(n+1)/2 orbital-pair densities
+// D^rs = (C_r C_s^T + C_s C_r^T)/2, which gives J^rs = (..|rs) in the AO
+// basis; W^rs = J^rs C^T is kept and reused by the gradient.  The Coulomb and
+// exchange contractions of every determinant pair are two dense products
+// [pairs x n^2] x [n^2 x n^2].
+//
+// Adjoint (one reverse sweep over the same determinant pairs and the kept
+// W^rs; no integral is stored or recomputed twice):
+//   dE/dGamma^s = (omega/N) (h + J(Gamma^t) - K(Gamma^s)),
+//   dE/dA = ((Phi - E_el)/N) omega G^T - G^T Gbar G^T   -> dE/dS_orb,
+//   dE/dh_orb = sum (omega/N) Gamma^t,  dE/d(pq|rs) = D2 = sum (omega/N)
+//   [Gamma^t (x) Gamma^t / 2 - sum_s Gamma^s (x) Gamma^s (exchange order) / 2],
+//   dE/dC = (Shat + Shat^T) C S_ao + (hhat + hhat^T) C h_ao + 4-index term
+//   through W^rs.
+#include "vsvb_internal.h"
+
+using namespace vsvb;
+
+namespace {
+
+constexpr int kMaxSpinDim = 60;  // largest alpha / beta block (LDS of one workgroup)
+
+// D[rs][mu][nu] = (C_r[mu] C_s[nu] + C_s[mu] C_r[nu]) / 2 for the unique orbital pairs.
+__global__ void k_pair_dens(const double* __restrict__ Cs, const int* __restrict__ pq, int npq, int nbf,
+                            double* __restrict__ D) {
+  const size_t nn = (size_t)nbf * nbf;
+  const size_t tot = (size_t)npq * nn;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const int w = (int)(t / nn);
+    const int mu = (int)((t % nn) / nbf), nu = (int)(t % nbf);
+    const int r = pq[2 * w], s = pq[2 * w + 1];
+    const double* cr = Cs + (size_t)r * nbf;
+    const double* cs = Cs + (size_t)s * nbf;
+    D[t] = 0.5 * (cr[mu] * cs[nu] + cs[mu] * cr[nu]);
+  }
+}
+
+// J = A + A^T for each of nmat n x n accumulators, in place.
+__global__ void k_sym_batch(double* __restrict__ A, int nmat, int n) {
+  const size_t nn = (size_t)n * n;
+  const size_t tot = (size_t)nmat * nn;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t w = t / nn;
+    const int i = (int)((t % nn) / n), j = (int)(t % n);
+    if (j > i) continue;
+    double* M = A + w * nn;
+    const double v = M[(size_t)i * n + j] + M[(size_t)j * n + i];
+    M[(size_t)i * n + j] = v;
+    M[(size_t)j * n + i] = v;
+  }
+}
+
+// (pq|rs) = sum_mu C[p][mu] W[rs][mu][q], stored for both (r,s) and (s,r):
+// eri[(p n + q) n^2 + r n + s].
+__global__ void k_orb_eri(const double* __restrict__ Cs, const double* __restrict__ W, const int* __restrict__ pq,
+                          int npq, int ns, int nbf, double* __restrict__ eri) {
+  const size_t n2 = (size_t)ns * ns;
+  const size_t tot = (size_t)npq * n2;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const int w = (int)(t / n2);
+    const int p = (int)((t % n2) / ns), q = (int)(t % ns);
+    const double* Wr = W + (size_t)w * nbf * ns;
+    const double* cp = Cs + (size_t)p * nbf;
+    double v = 0.0;
+    for (int mu = 0; mu < nbf; ++mu) v += cp[mu] * Wr[(size_t)mu * ns + q];
+    const int r = pq[2 * w], s = pq[2 * w + 1];
+    eri[((size_t)p * ns + q) * n2 + (size_t)r * ns + s] = v;
+    eri[((size_t)p * ns + q) * n2 + (size_t)s * ns + r] = v;
+  }
+}
+
+// Exchange ordering: erik[(a n + b) n^2 + c n + d] = (cb|ad).
+__global__ void k_orb_erik(const double* __restrict__ eri, int ns, double* __restrict__ erik) {
+  const size_t n2 = (size_t)ns * ns;
+  const size_t tot = n2 * n2;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const int ab = (int)(t / n2), cd = (int)(t % n2);
+    const int a = ab / ns, b = ab % ns, c = cd / ns, d = cd % ns;
+    erik[t] = eri[((size_t)c * ns + b) * n2 + (size_t)a * ns + d];
+  }
+}
+
+// Gauss-Jordan inverse with partial pivoting of the n x n matrix held in the
+// left half of aug (n x 2n, LDS).  Returns det (0 when a pivot vanishes).
+__device__ double gj_inverse(double* aug, double* colk, int n, int* pivrow) {
+  const int m = 2 * n;
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t % n;
+    aug[(size_t)i * m + n + j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double det = 1.0;
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      int best = k;
+      double bv = fabs(aug[(size_t)k * m + k]);
+      for (int i = k + 1; i < n; ++i) {
+        const double v = fabs(aug[(size_t)i * m + k]);
+        if (v > bv) { bv = v; best = i; }
+      }
+      *pivrow = best;
+    }
+    __syncthreads();
+    const int pr = *pivrow;
+    if (pr != k) {
+      for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        const double x = aug[(size_t)k * m + j];
+        aug[(size_t)k * m + j] = aug[(size_t)pr * m + j];
+        aug[(size_t)pr * m + j] = x;
+      }
+      det = -det;
+    }
+    __syncthreads();
+    const double piv = aug[(size_t)k * m + k];
+    det *= piv;
+    if (piv == 0.0 || !isfinite(piv)) return 0.0;  // uniform: every thread read the same pivot
+    const double ip = 1.0 / piv;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = aug[(size_t)i * m + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) aug[(size_t)k * m + j] *= ip;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n * m; t += blockDim.x) {
+      const int i = t / m, j = t % m;
+      if (i == k) continue;
+      aug[t] -= colk[i] * aug[(size_t)k * m + j];
+    }
+    __syncthreads();
+  }
+  return det;
+}
+
+struct DetArgs {
+  const double* Sorb;
+  const int* alist;   // [ndet][na]
+  const int* blist;   // [ndet][nb]
+  const double* cdet; // [ndet]
+  int ndet, na, nb, ns;
+  double* Ga;         // [pairs][na^2]  (G[c][r])
+  double* Gb;
+  double* Xa;         // [pairs][ns^2]
+  double* Xb;
+  double* Xt;
+  double* w;          // [pairs]
+  int* status;        // status[2] = singular transition overlap
+};
+
+// One workgroup per ordered determinant pair t = S * ndet + T.
+__global__ void __launch_bounds__(64) k_det_pairs(DetArgs a, long p0) {
+  extern __shared__ double lds[];
+  __shared__ int pivrow;
+  const long t = p0 + blockIdx.x;
+  const int S = (int)(t / a.ndet), T = (int)(t % a.ndet);
+  const int ns = a.ns, n2 = ns * ns;
+  const int dmax = a.na > a.nb ? a.na : a.nb;
+  double* aug = lds;
+  double* colk = lds + 2 * dmax * dmax;
+  double* Xa = a.Xa + (size_t)t * n2;
+  double* Xb = a.Xb + (size_t)t * n2;
+  double* Xt = a.Xt + (size_t)t * n2;
+  for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+    Xa[k] = 0.0;
+    Xb[k] = 0.0;
+    Xt[k] = 0.0;
+  }
+  double dets[2] = {1.0, 1.0};
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin == 0 ? a.na : a.nb;
+    if (n == 0) continue;
+    const int* lb = (spin == 0 ? a.alist : a.blist) + (size_t)S * n;
+    const int* lk = (spin == 0 ? a.alist : a.blist) + (size_t)T * n;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int r = k / n, c = k % n;
+      aug[(size_t)r * 2 * n + c] = a.Sorb[(size_t)lb[r] * ns + lk[c]];
+    }
+    __syncthreads();
+    const double d = gj_inverse(aug, colk, n, &pivrow);
+    dets[spin] = d;
+    if (d == 0.0) {
+      if (threadIdx.x == 0) atomicOr(a.status + 2, 1);
+      continue;
+    }
+    double* G = (spin == 0 ? a.Ga + (size_t)t * a.na * a.na : a.Gb + (size_t)t * a.nb * a.nb);
+    double* X = spin == 0 ? Xa : Xb;
+    // inverse = right half; G[c][r] pairs with Gamma[bra orbital of r][ket orbital of c]
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int c = k / n, r = k % n;
+      const double g = aug[(size_t)c * 2 * n + n + r];
+      G[k] = g;
+      X[(size_t)lb[r] * ns + lk[c]] = g;
+    }
+    __syncthreads();
+    // Gamma^t accumulates alpha then beta (a docc orbital sits in both lists)
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int c = k / n, r = k % n;
+      const size_t pos = (size_t)lb[r] * ns + lk[c];
+      Xt[pos] += aug[(size_t)c * 2 * n + n + r];
+    }
+  }
+  if (threadIdx.x == 0) a.w[t] = a.cdet[S] * a.cdet[T] * dets[0] * dets[1];
+}
+
+__device__ __forceinline__ double block_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Phi = e1 + e2 per determinant pair (one wavefront each); num = omega * Phi, e1w = omega * e1.
+__global__ void __launch_bounds__(64) k_pair_energy(const double* __restrict__ h, const double* __restrict__ Xa,
+                                                    const double* __restrict__ Xb, const double* __restrict__ Xt,
+                                                    const double* __restrict__ JX, const double* __restrict__ KXa,
+                                                    const double* __restrict__ KXb, const double* __restrict__ w,
+                                                    int n2, long p0, double* __restrict__ phi,
+                                                    double* __restrict__ num, double* __restrict__ e1w) {
+  const long t = p0 + blockIdx.x;
+  const size_t o = (size_t)t * n2;
+  double e1 = 0.0, e2 = 0.0;
+  for (int k = threadIdx.x; k < n2; k += 64) {
+    const double xt = Xt[o + k];
+    e1 += h[k] * xt;
+    e2 += 0.5 * xt * JX[o + k] - 0.5 * (Xa[o + k] * KXa[o + k] + Xb[o + k] * KXb[o + k]);
+  }
+  e1 = block_sum64(e1);
+  e2 = block_sum64(e2);
+  if (threadIdx.x == 0) {
+    phi[t] = e1 + e2;
+    num[t] = w[t] * (e1 + e2);
+    e1w[t] = w[t] * e1;
+  }
+}
+
+// scal[0] = sum omega Phi, scal[1] = N = sum omega, scal[4] = sum omega e1 (from the reductions);
+// scal[2] = E, scal[3] = E_el, scal[5] = E1.
+__global__ void k_sc_final(double* __restrict__ scal, double enuc) {
+  const double el = scal[0] / scal[1];
+  scal[3] = el;
+  scal[2] = el + enuc;
+  scal[5] = scal[4] / scal[1];
+}
+
+struct AdjArgs {
+  const double* h;
+  const double* JX;
+  const double* KXa;
+  const double* KXb;
+  const double* Ga;
+  const double* Gb;
+  const double* w;
+  const double* phi;
+  const double* scal;
+  const int* alist;
+  const int* blist;
+  int ndet, na, nb, ns;
+  double* Shat;  // [ns^2] dE/dS_orb (atomic)
+  double* wv;    // [pairs] omega / N
+};
+
+// Adjoint of one determinant pair: Abar = ((Phi - E_el)/N) omega G^T - G^T Gbar G^T,
+// Gbar[c][r] = (omega/N) (h + J - K_s)[p_r][q_c]; scattered into Shat.
+__global__ void __launch_bounds__(64) k_pair_adjoint(AdjArgs a, long p0) {
+  extern __shared__ double lds[];
+  const long t = p0 + blockIdx.x;
+  const int S = (int)(t / a.ndet), T = (int)(t % a.ndet);
+  const int ns = a.ns, n2 = ns * ns;
+  const double N = a.scal[1], el = a.scal[3];
+  const double om = a.w[t];
+  const double f = om / N, g = (a.phi[t] - el) / N * om;
+  if (threadIdx.x == 0) a.wv[t] = f;
+  if (om == 0.0) return;
+  const int dmax = a.na > a.nb ? a.na : a.nb;
+  double* G = lds;                    // n x n
+  double* Gb = lds + dmax * dmax;     // Gbar
+  double* M1 = lds + 2 * dmax * dmax; // Gbar G^T
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin == 0 ? a.na : a.nb;
+    if (n == 0) continue;
+    const int* lb = (spin == 0 ? a.alist : a.blist) + (size_t)S * n;
+    const int* lk = (spin == 0 ? a.alist : a.blist) + (size_t)T * n;
+    const double* Gg = (spin == 0 ? a.Ga + (size_t)t * a.na * a.na : a.Gb + (size_t)t * a.nb * a.nb);
+    const double* KX = (spin == 0 ? a.KXa : a.KXb) + (size_t)t * n2;
+    const double* JX = a.JX + (size_t)t * n2;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int c = k / n, r = k % n;
+      const size_t pos = (size_t)lb[r] * ns + lk[c];
+      G[k] = Gg[k];
+      Gb[k] = f * (a.h[pos] + JX[pos] - KX[pos]);
+    }
+    __syncthreads();
+    // M1[i][c] = sum_j Gbar[i][j] G[c][j]
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int i = k / n, c = k % n;
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += Gb[i * n + j] * G[c * n + j];
+      M1[k] = s;
+    }
+    __syncthreads();
+    // Abar[r][c] = g G[c][r] - sum_i G[i][r] M1[i][c]
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+      const int r = k / n, c = k % n;
+      double s = g * G[c * n + r];
+      for (int i = 0; i < n; ++i) s -= G[i * n + r] * M1[i * n + c];
+      atomicAdd(&a.Shat[(size_t)lb[r] * ns + lk[c]], s);
+    }
+  }
+}
+
+// Rows of X scaled by wv: Y[t][k] = wv[t] X[t][k].
+__global__ void k_scale_rows(const double* __restrict__ X, const double* __restrict__ wv, long rows, int n2,
+                             double* __restrict__ Y) {
+  const size_t tot = (size_t)rows * n2;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x)
+    Y[t] = wv[t / n2] * X[t];
+}
+
+// D2[(p q),(r s)] -= 1/2 Dk[(p s),(r q)]
+__global__ void k_exchange_fold(const double* __restrict__ Dk, int ns, double* __restrict__ D2) {
+  const size_t n2 = (size_t)ns * ns;
+  const size_t tot = n2 * n2;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const int pq = (int)(t / n2), rs = (int)(t % n2);
+    const int p = pq / ns, q = pq % ns, r = rs / ns, s = rs % ns;
+    D2[t] -= 0.5 * Dk[((size_t)p * ns + s) * n2 + (size_t)r * ns + q];
+  }
+}
+
+// Dh[t][q][w] for the unique orbital pair w = (r, s): the coefficient of
+// W^rs[mu][q] in dE/dC[t][mu]; Dtil[tq][rs] = D2[tq,rs] + D2[qt,rs] + D2[rs,tq] + D2[rs,qt],
+// summed over both orders of (r, s) (W^rs = W^sr).
+__global__ void k_grad_coef(const double* __restrict__ D2, const int* __restrict__ pq, int npq, int ns,
+                            double* __restrict__ Dh) {
+  const size_t n2 = (size_t)ns * ns;
+  const size_t tot = n2 * npq;
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot; x += (size_t)gridDim.x * blockDim.x) {
+    const int tq = (int)(x / npq), w = (int)(x % npq);
+    const int t = tq / ns, q = tq % ns;
+    const int r = pq[2 * w], s = pq[2 * w + 1];
+    auto dt = [&](int a, int b) {
+      const size_t ab = (size_t)a * ns + b;
+      const size_t tq1 = (size_t)t * ns + q, qt1 = (size_t)q * ns + t;
+      return D2[tq1 * n2 + ab] + D2[qt1 * n2 + ab] + D2[ab * n2 + tq1] + D2[ab * n2 + qt1];
+    };
+    double v = dt(r, s);
+    if (r != s) v += dt(s, r);
+    Dh[x] = v;
+  }
+}
+
+// gradC[t][mu] = sum_w sum_q Dh[t][q][w] W[w][mu][q]
+__global__ void k_grad_eri(const double* __restrict__ Dh, const double* __restrict__ W, int npq, int ns, int nbf,
+                           double* __restrict__ gradC) {
+  const size_t tot = (size_t)ns * nbf;
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot; x += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(x / nbf), mu = (int)(x % nbf);
+    double v = 0.0;
+    for (int w = 0; w < npq; ++w) {
+      const double* Wm = W + ((size_t)w * nbf + mu) * ns;
+      for (int q = 0; q < ns; ++q) v += Dh[((size_t)t * ns + q) * npq + w] * Wm[q];
+    }
+    gradC[x] = v;
+  }
+}
+
+// M = X + X^T (n x n)
+__global__ void k_sym_into(const double* __restrict__ X, int n, double* __restrict__ M) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t / n, j = t % n;
+  M[t] = X[t] + X[(size_t)j * n + i];
+}
+
+unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(4096, (n + 255) / 256 + 1); }
+
+// op(A)[M x K] * op(B)[K x N] with M split into chunks the 2-D grid can hold.
+hipError_t gemm_rows(hipStream_t st, long M, int N, int K, double alpha, const double* A, int lda,
+                     const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
+  const long chunk = 65535L * 16;
+  for (long r0 = 0; r0 < M; r0 += chunk) {
+    const int m = (int)std::min(chunk, M - r0);
+    const hipError_t e = gemm(st, m, N, K, alpha, A + (size_t)r0 * lda, lda, false, B, ldb, tb, beta,
+                              C + (size_t)r0 * ldc, ldc);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
+  SpinCoupled& sc = c->sc;
+  if (s->nspinc < 1 || !s->pair_sc || !s->coeff_sc) return fail(VSVB_ERR_ARG, "spin-coupled pairs without spin couplings");
+  if (s->npair > 16) return fail(VSVB_ERR_UNSUPPORTED, "more than 16 spin-coupled pairs");
+  const int np = s->npair, nnd = 2 * s->npair + s->nunpd;
+  sc.na = s->npair + s->nunpd + s->ndocc;
+  sc.nb = s->npair + s->ndocc;
+  if (sc.na > kMaxSpinDim) return fail(VSVB_ERR_UNSUPPORTED, "alpha block larger than 60 orbitals");
+  sc.ndet = s->nspinc << np;
+  sc.npairs = (long)sc.ndet * sc.ndet;
+  const int ns = c->nsp;
+  sc.npq = ns * (ns + 1) / 2;
+  std::vector<int> al((size_t)sc.ndet * sc.na), bl((size_t)sc.ndet * std::max(sc.nb, 1)), pq(2 * (size_t)sc.npq);
+  std::vector<double> cd(sc.ndet);
+  for (int isc = 0; isc < s->nspinc; ++isc) {
+    std::vector<int> seen(nnd, 0);
+    for (int k = 0; k < np; ++k)
+      for (int ab = 0; ab < 2; ++ab) {
+        const int slot = s->pair_sc[((size_t)isc * np + k) * 2 + ab];
+        if (slot < 1 || slot > 2 * np) return fail(VSVB_ERR_ARG, "pair_sc slot outside the spin-coupled orbitals");
+        if (seen[slot - 1]++) return fail(VSVB_ERR_ARG, "pair_sc uses an orbital twice in one spin coupling");
+      }
+    for (int mask = 0; mask < (1 << np); ++mask) {
+      const int d = (isc << np) + mask;
+      int* A = al.data() + (size_t)d * sc.na;
+      int* B = bl.data() + (size_t)d * sc.nb;
+      for (int k = 0; k < np; ++k) {
+        int x = s->pair_sc[((size_t)isc * np + k) * 2] - 1, y = s->pair_sc[((size_t)isc * np + k) * 2 + 1] - 1;
+        if (mask >> k & 1) std::swap(x, y);
+        A[k] = x;
+        B[k] = y;
+      }
+      for (int u = 0; u < s->nunpd; ++u) A[np + u] = 2 * np + u;
+      for (int o = 0; o < s->ndocc; ++o) {
+        A[np + s->nunpd + o] = nnd + o;
+        B[np + o] = nnd + o;
+      }
+      cd[d] = s->coeff_sc[isc];
+    }
+  }
+  for (int r = 0, w = 0; r < ns; ++r)
+    for (int q = 0; q <= r; ++q, ++w) {
+      pq[2 * w] = r;
+      pq[2 * w + 1] = q;
+    }
+  const size_t n2 = (size_t)ns * ns, nn = (size_t)c->nbf * c->nbf, P = (size_t)sc.npairs;
+  const double bytes = 8.0 * (2.0 * sc.npq * nn + (double)sc.npq * c->nbf * ns + 4.0 * n2 * n2 +
+                              P * (sc.na * sc.na + sc.nb * sc.nb + 7.0 * n2 + 4.0) + n2 * ns * sc.npq);
+  if (bytes > 64e9) return fail(VSVB_ERR_UNSUPPORTED, "spin-coupled work arrays would exceed 64 GB");
+  HIP_OK(sc.d_alist.alloc(al.size()));
+  HIP_OK(sc.d_blist.alloc(bl.size()));
+  HIP_OK(sc.d_cdet.alloc(cd.size()));
+  HIP_OK(sc.d_pq.alloc(pq.size()));
+  HIP_OK(sc.d_pdens.alloc(sc.npq * nn));
+  HIP_OK(sc.d_pj.alloc(sc.npq * nn));
+  HIP_OK(sc.d_W.alloc((size_t)sc.npq * c->nbf * ns));
+  HIP_OK(sc.d_eri.alloc(n2 * n2));
+  HIP_OK(sc.d_erik.alloc(n2 * n2));
+  HIP_OK(sc.d_Ga.alloc(P * sc.na * sc.na));
+  HIP_OK(sc.d_Gb.alloc(P * std::max(sc.nb * sc.nb, 1)));
+  HIP_OK(sc.d_Xa.alloc(P * n2));
+  HIP_OK(sc.d_Xb.alloc(P * n2));
+  HIP_OK(sc.d_Xt.alloc(P * n2));
+  HIP_OK(sc.d_JX.alloc(P * n2));
+  HIP_OK(sc.d_KXa.alloc(P * n2));
+  HIP_OK(sc.d_KXb.alloc(P * n2));
+  HIP_OK(sc.d_w.alloc(P));
+  HIP_OK(sc.d_num.alloc(P));
+  HIP_OK(sc.d_phi.alloc(P));
+  HIP_OK(sc.d_e1w.alloc(P));
+  HIP_OK(sc.d_wv.alloc(P));
+  HIP_OK(sc.d_Shat.alloc(n2));
+  HIP_OK(sc.d_hhat.alloc(n2));
+  HIP_OK(sc.d_hsym.alloc(n2));
+  HIP_OK(sc.d_D2.alloc(n2 * n2));
+  HIP_OK(sc.d_Dk.alloc(n2 * n2));
+  HIP_OK(sc.d_Dh.alloc(n2 * sc.npq));
+  HIP_OK(sc.d_Xw.alloc(P * n2));
+  HIP_OK(hipMemcpy(sc.d_alist.p, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sc.d_blist.p, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sc.d_cdet.p, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sc.d_pq.p, pq.data(), pq.size() * sizeof(int), hipMemcpyHostToDevice));
+  return VSVB_OK;
+}
+
+// Energy (and gradient when grad_dev) of a spin-coupled wave function; the
+// geometry and orbital basis (C, C_eri, S_orb, h_orb) are prepared by the caller.
+int vsvb::sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev) {
+  SpinCoupled& sc = c->sc;
+  hipStream_t st = c->stream;
+  const int ns = c->nsp, nbf = c->nbf, n2 = ns * ns;
+  const size_t nn = (size_t)nbf * nbf;
+  const long P = sc.npairs;
+  int rc;
+  // orbital-pair Coulomb matrices from one pass over the screened shell quartets
+  hipLaunchKernelGGL(k_pair_dens, dim3(grid_for(sc.npq * nn)), dim3(256), 0, st, c->d_Cs.p, sc.d_pq.p, sc.npq, nbf,
+                     sc.d_pdens.p);
+  HIP_OK(hipGetLastError());
+  if ((rc = run_quartets(c, true, nullptr, sc.npq, sc.d_pdens.p, sc.d_pj.p, c->d_K.p, 0))) return rc;
+  hipLaunchKernelGGL(k_sym_batch, dim3(grid_for(sc.npq * nn)), dim3(256), 0, st, sc.d_pj.p, sc.npq, nbf);
+  HIP_OK(hipGetLastError());
+  // W^rs = J^rs C^T  ([npq nbf] x nbf times nbf x ns), then (pq|rs) and its exchange ordering
+  HIP_OK(gemm_rows(st, (long)sc.npq * nbf, ns, nbf, 1.0, sc.d_pj.p, nbf, c->d_Cs.p, nbf, true, 0.0, sc.d_W.p, ns));
+  hipLaunchKernelGGL(k_orb_eri, dim3(grid_for((size_t)sc.npq * n2)), dim3(256), 0, st, c->d_Cs.p, sc.d_W.p,
+                     sc.d_pq.p, sc.npq, ns, nbf, sc.d_eri.p);
+  hipLaunchKernelGGL(k_orb_erik, dim3(grid_for((size_t)n2 * n2)), dim3(256), 0, st, sc.d_eri.p, ns, sc.d_erik.p);
+  HIP_OK(hipGetLastError());
+  // determinant pairs: inverses, determinants, transition densities
+  DetArgs da{c->d_Sorb.p, sc.d_alist.p, sc.d_blist.p, sc.d_cdet.p, sc.ndet, sc.na, sc.nb, ns,
+             sc.d_Ga.p, sc.d_Gb.p, sc.d_Xa.p, sc.d_Xb.p, sc.d_Xt.p, sc.d_w.p, c->d_status.p};
+  const int dmax = std::max(sc.na, sc.nb);
+  const size_t lds_det = (2 * (size_t)dmax * dmax + dmax) * sizeof(double);
+  const long chunk = 1L << 30;
+  for (long p0 = 0; p0 < P; p0 += chunk) {
+    const unsigned nb = (unsigned)std::min(chunk, P - p0);
+    hipLaunchKernelGGL(k_det_pairs, dim3(nb), dim3(64), lds_det, st, da, p0);
+    HIP_OK(hipGetLastError());
+  }
+  // Coulomb and exchange contractions of every pair: [P x n^2] x [n^2 x n^2]
+  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xt.p, n2, sc.d_eri.p, n2, false, 0.0, sc.d_JX.p, n2));
+  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xa.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KXa.p, n2));
+  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xb.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KXb.p, n2));
+  for (long p0 = 0; p0 < P; p0 += chunk) {
+    const unsigned nb = (unsigned)std::min(chunk, P - p0);
+    hipLaunchKernelGGL(k_pair_energy, dim3(nb), dim3(64), 0, st, c->d_Horb.p, sc.d_Xa.p, sc.d_Xb.p, sc.d_Xt.p,
+                       sc.d_JX.p, sc.d_KXa.p, sc.d_KXb.p, sc.d_w.p, n2, p0, sc.d_phi.p, sc.d_num.p, sc.d_e1w.p);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(launch_reduce(st, sc.d_num.p, P, 1, c->d_scal.p + 0));
+  HIP_OK(launch_reduce(st, sc.d_w.p, P, 1, c->d_scal.p + 1));
+  HIP_OK(launch_reduce(st, sc.d_e1w.p, P, 1, c->d_scal.p + 4));
+  hipLaunchKernelGGL(k_sc_final, dim3(1), dim3(1), 0, st, c->d_scal.p, c->enuc);
+  HIP_OK(hipGetLastError());
+  if (!want_grad) return VSVB_OK;
+
+  // ---- adjoint
+  HIP_OK(launch_fill(st, sc.d_Shat.p, n2, 0.0));
+  AdjArgs aa{c->d_Horb.p, sc.d_JX.p, sc.d_KXa.p, sc.d_KXb.p, sc.d_Ga.p, sc.d_Gb.p, sc.d_w.p, sc.d_phi.p,
+             c->d_scal.p, sc.d_alist.p, sc.d_blist.p, sc.ndet, sc.na, sc.nb, ns, sc.d_Shat.p, sc.d_wv.p};
+  const size_t lds_adj = 3 * (size_t)dmax * dmax * sizeof(double);
+  for (long p0 = 0; p0 < P; p0 += chunk) {
+    const unsigned nb = (unsigned)std::min(chunk, P - p0);
+    hipLaunchKernelGGL(k_pair_adjoint, dim3(nb), dim3(64), lds_adj, st, aa, p0);
+    HIP_OK(hipGetLastError());
+  }
+  // hhat = sum_t wv_t Gamma^t_t ; D2 = 1/2 Xw^T Xt - 1/2 (exchange-ordered Xwa^T Xa + Xwb^T Xb)
+  HIP_OK(gemm(st, 1, n2, (int)P, 1.0, sc.d_wv.p, (int)P, false, sc.d_Xt.p, n2, false, 0.0, sc.d_hhat.p, n2));
+  hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(P * n2)), dim3(256), 0, st, sc.d_Xt.p, sc.d_wv.p, P, n2, sc.d_Xw.p);
+  HIP_OK(gemm(st, n2, n2, (int)P, 0.5, sc.d_Xw.p, n2, true, sc.d_Xt.p, n2, false, 0.0, sc.d_D2.p, n2));
+  for (int spin = 0; spin < 2; ++spin) {
+    const double* X = spin == 0 ? sc.d_Xa.p : sc.d_Xb.p;
+    hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(P * n2)), dim3(256), 0, st, X, sc.d_wv.p, P, n2, sc.d_Xw.p);
+    HIP_OK(gemm(st, n2, n2, (int)P, 1.0, sc.d_Xw.p, n2, true, X, n2, false, 0.0, sc.d_Dk.p, n2));
+    hipLaunchKernelGGL(k_exchange_fold, dim3(grid_for((size_t)n2 * n2)), dim3(256), 0, st, sc.d_Dk.p, ns, sc.d_D2.p);
+    HIP_OK(hipGetLastError());
+  }
+  // four-index term through W^rs, then the overlap and one-electron terms
+  hipLaunchKernelGGL(k_grad_coef, dim3(grid_for((size_t)n2 * sc.npq)), dim3(256), 0, st, sc.d_D2.p, sc.d_pq.p,
+                     sc.npq, ns, sc.d_Dh.p);
+  hipLaunchKernelGGL(k_grad_eri, dim3(grid_for((size_t)ns * nbf)), dim3(256), 0, st, sc.d_Dh.p, sc.d_W.p, sc.npq, ns,
+                     nbf, c->d_gradC.p);
+  HIP_OK(hipGetLastError());
+  for (int k = 0; k < 2; ++k) {
+    const double* X = k == 0 ? sc.d_Shat.p : sc.d_hhat.p;
+    const double* M = k == 0 ? c->d_S.p : c->d_H.p;
+    hipLaunchKernelGGL(k_sym_into, dim3((n2 + 255) / 256), dim3(256), 0, st, X, ns, sc.d_hsym.p);
+    HIP_OK(gemm(st, ns, nbf, ns, 1.0, sc.d_hsym.p, ns, false, c->d_Ct.p, nbf, false, 0.0, c->d_X.p, nbf));
+    HIP_OK(gemm(st, ns, nbf, nbf, 1.0, c->d_X.p, nbf, false, M, nbf, false, 1.0, c->d_gradC.p, nbf));
+  }
+  return launch_gather_grad(c, c->d_gradC.p, grad_dev);
+}
+
+int vsvb::sc_finish(vsvb_context* c, double* energy, double* parts, double* grad_host) {
+  int rc;
+  double h[6];
+  HIP_OK(hipMemcpyAsync(h, c->d_scal.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (grad_host)
+    HIP_OK(hipMemcpyAsync(grad_host, c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = check_status(c, grad_host != nullptr))) return rc;
+  if ((rc = finish_counters(c))) return rc;
+  if (!std::isfinite(h[2])) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
+  if (energy) *energy = h[2];
+  if (parts) {
+    parts[0] = h[3];
+    parts[1] = 1.0;
+    parts[2] = h[5];
+    parts[3] = c->enuc;
+  }
+  return VSVB_OK;
+}
