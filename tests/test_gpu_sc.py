@@ -36,15 +36,12 @@ def _load(key):
 
 
 SC = sorted(k for k in ORACLE_KNOWN if _load(k).npair > 0)
-# Determinant pairs of this expansion have exactly singular transition overlaps
-# (orthogonal orbitals by symmetry; corank 1 and 2).
-SINGULAR = {"testing/n2.sc4val-b.p2"}
+# testing/n2.sc4val-b.p2 has determinant pairs whose transition overlaps are
+# exactly singular (orthogonal orbitals by symmetry; corank 1 and 2).
 
 
 @pytest.mark.parametrize("key", SC)
 def test_sc_known_answers_gpu(lib, key):
-    if key in SINGULAR:
-        pytest.skip("singular transition overlaps: cofactors of singular blocks not implemented on the GPU path")
     grp, name = key.split("/")
     print(f"case {key}", flush=True)
     s = _load(key)

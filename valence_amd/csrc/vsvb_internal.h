@@ -81,6 +81,8 @@ struct SpinCoupled {
   DevBuf<double> d_Shat, d_hhat, d_hsym;          // [nsp^2] dE/dS_orb, dE/dh_orb, symmetrized copy
   DevBuf<double> d_D2, d_Dk;                      // [nsp^2][nsp^2] dE/d(pq|rs), exchange outer product
   DevBuf<double> d_Dh, d_Xw;                      // [nsp^2][npq], [npairs][nsp^2] adjoint work
+  DevBuf<double> d_Y, d_JY, d_KY;                 // [npairs][2][3][nsp^2] split cofactor matrices and images
+  DevBuf<double> d_q4;                            // [npairs][2][4] sign, P_R, d1, d2
 };
 
 }  // namespace vsvb

@@ -36,7 +36,7 @@ using namespace vsvb;
 
 namespace {
 
-constexpr int kMaxSpinDim = 60;  // largest alpha / beta block (LDS of one workgroup)
+constexpr int kMaxSpinDim = 48;  // largest alpha / beta block (three n x n LDS arrays per workgroup)
 
 // D[rs][mu][nu] = (C_r[mu] C_s[nu] + C_s[mu] C_r[nu]) / 2 for the unique orbital pairs.
 __global__ void k_pair_dens(const double* __restrict__ Cs, const int* __restrict__ pq, int npq, int nbf,
@@ -98,124 +98,167 @@ __global__ void k_orb_erik(const double* __restrict__ eri, int ns, double* __res
   }
 }
 
-// Gauss-Jordan inverse with partial pivoting of the n x n matrix held in the
-// left half of aug (n x 2n, LDS).  Returns det (0 when a pivot vanishes).
-__device__ double gj_inverse(double* aug, double* colk, int n, int* pivrow) {
-  const int m = 2 * n;
-  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-    const int i = t / n, j = t % n;
-    aug[(size_t)i * m + n + j] = i == j ? 1.0 : 0.0;
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o, 64);
+    const int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
   }
-  __syncthreads();
-  double det = 1.0;
-  for (int k = 0; k < n; ++k) {
-    if (threadIdx.x == 0) {
-      int best = k;
-      double bv = fabs(aug[(size_t)k * m + k]);
-      for (int i = k + 1; i < n; ++i) {
-        const double v = fabs(aug[(size_t)i * m + k]);
-        if (v > bv) { bv = v; best = i; }
-      }
-      *pivrow = best;
-    }
-    __syncthreads();
-    const int pr = *pivrow;
-    if (pr != k) {
-      for (int j = threadIdx.x; j < m; j += blockDim.x) {
-        const double x = aug[(size_t)k * m + j];
-        aug[(size_t)k * m + j] = aug[(size_t)pr * m + j];
-        aug[(size_t)pr * m + j] = x;
-      }
-      det = -det;
-    }
-    __syncthreads();
-    const double piv = aug[(size_t)k * m + k];
-    det *= piv;
-    if (piv == 0.0 || !isfinite(piv)) return 0.0;  // uniform: every thread read the same pivot
-    const double ip = 1.0 / piv;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = aug[(size_t)i * m + k];
-    __syncthreads();
-    for (int j = threadIdx.x; j < m; j += blockDim.x) aug[(size_t)k * m + j] *= ip;
-    __syncthreads();
-    for (int t = threadIdx.x; t < n * m; t += blockDim.x) {
-      const int i = t / m, j = t % m;
-      if (i == k) continue;
-      aug[t] -= colk[i] * aug[(size_t)k * m + j];
-    }
-    __syncthreads();
-  }
-  return det;
 }
 
 struct DetArgs {
   const double* Sorb;
-  const int* alist;   // [ndet][na]
-  const int* blist;   // [ndet][nb]
-  const double* cdet; // [ndet]
+  const int* alist;    // [ndet][na]
+  const int* blist;    // [ndet][nb]
+  const double* cdet;  // [ndet]
   int ndet, na, nb, ns;
-  double* Ga;         // [pairs][na^2]  (G[c][r])
-  double* Gb;
-  double* Xa;         // [pairs][ns^2]
-  double* Xb;
-  double* Xt;
-  double* w;          // [pairs]
-  int* status;        // status[2] = singular transition overlap
+  double* Y;           // [pairs][2 spins][3][ns^2]: Gamma_R, g1, g2 in orbital space
+  double* q4;          // [pairs][2 spins][4]: s, P, d1, d2
+  int* status;         // status[3]: a block with a pivot ratio below 1e-9 (gradient refused)
 };
 
-// One workgroup per ordered determinant pair t = S * ndet + T.
+// Cofactors of a possibly singular block without an inverse.  Complete-
+// pivoting LU, P A Q = L D U~ (L, U~ unit triangular), so that
+//   A^-1 = Q U~^-1 D^-1 L^-1 P   wherever it exists.
+// With z1 = n-1, z2 = n-2 (the two smallest pivots under complete pivoting),
+// R = the other pivots, P_R = prod_R d_k, s = sign of P and Q:
+//   Gamma_R = sum_{k in R} U~^-1[:,k] L^-1[k,:] / d_k,  g_z = U~^-1[:,z] L^-1[z,:],
+//   det = s d1 d2 P_R,  first-order cofactors = s P_R (d1 d2 Gamma_R + d2 g1 + d1 g2),
+//   second-order cofactors = s P_R [d1 d2 Gamma_R(x)Gamma_R + d2 (g1(x)Gamma_R + Gamma_R(x)g1)
+//                                   + d1 (g2(x)Gamma_R + Gamma_R(x)g2) + g1(x)g2 + g2(x)g1]
+// (antisymmetrized pairs; the self products of the rank-one g_z cancel).  These
+// are the cofactors of the diagonal factor carried through X = P^T L and
+// Y^T = U~ Q^T, polynomial in the pivots, so they hold for d1 = 0 or d2 = 0;
+// with a zero pivot in R every cofactor up to second order vanishes.
+// One workgroup (one wavefront) per ordered determinant pair t = S * ndet + T.
 __global__ void __launch_bounds__(64) k_det_pairs(DetArgs a, long p0) {
   extern __shared__ double lds[];
-  __shared__ int pivrow;
+  __shared__ int prow[kMaxSpinDim], pcol[kMaxSpinDim];
+  __shared__ double piv[kMaxSpinDim];
   const long t = p0 + blockIdx.x;
   const int S = (int)(t / a.ndet), T = (int)(t % a.ndet);
   const int ns = a.ns, n2 = ns * ns;
   const int dmax = a.na > a.nb ? a.na : a.nb;
-  double* aug = lds;
-  double* colk = lds + 2 * dmax * dmax;
-  double* Xa = a.Xa + (size_t)t * n2;
-  double* Xb = a.Xb + (size_t)t * n2;
-  double* Xt = a.Xt + (size_t)t * n2;
-  for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-    Xa[k] = 0.0;
-    Xb[k] = 0.0;
-    Xt[k] = 0.0;
-  }
-  double dets[2] = {1.0, 1.0};
+  const int tid = threadIdx.x;
+  double* M = lds;                     // n x n, factored in place
+  double* Ui = lds + dmax * dmax;      // U~^-1
+  double* Li = lds + 2 * dmax * dmax;  // L^-1
   for (int spin = 0; spin < 2; ++spin) {
     const int n = spin == 0 ? a.na : a.nb;
-    if (n == 0) continue;
-    const int* lb = (spin == 0 ? a.alist : a.blist) + (size_t)S * n;
-    const int* lk = (spin == 0 ? a.alist : a.blist) + (size_t)T * n;
-    __syncthreads();
-    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-      const int r = k / n, c = k % n;
-      aug[(size_t)r * 2 * n + c] = a.Sorb[(size_t)lb[r] * ns + lk[c]];
-    }
-    __syncthreads();
-    const double d = gj_inverse(aug, colk, n, &pivrow);
-    dets[spin] = d;
-    if (d == 0.0) {
-      if (threadIdx.x == 0) atomicOr(a.status + 2, 1);
+    double* Yr = a.Y + ((size_t)t * 2 + spin) * 3 * n2;
+    double* Y1 = Yr + n2;
+    double* Y2 = Y1 + n2;
+    double* q = a.q4 + ((size_t)t * 2 + spin) * 4;
+    for (int k = tid; k < 3 * n2; k += 64) Yr[k] = 0.0;
+    if (n == 0) {
+      if (tid == 0) {
+        q[0] = 1.0; q[1] = 1.0; q[2] = 1.0; q[3] = 1.0;
+      }
       continue;
     }
-    double* G = (spin == 0 ? a.Ga + (size_t)t * a.na * a.na : a.Gb + (size_t)t * a.nb * a.nb);
-    double* X = spin == 0 ? Xa : Xb;
-    // inverse = right half; G[c][r] pairs with Gamma[bra orbital of r][ket orbital of c]
-    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-      const int c = k / n, r = k % n;
-      const double g = aug[(size_t)c * 2 * n + n + r];
-      G[k] = g;
-      X[(size_t)lb[r] * ns + lk[c]] = g;
+    const int* lb = (spin == 0 ? a.alist : a.blist) + (size_t)S * n;
+    const int* lk = (spin == 0 ? a.alist : a.blist) + (size_t)T * n;
+    __syncthreads();  // the previous spin block is done with the LDS
+    for (int k = tid; k < n * n; k += 64) M[k] = a.Sorb[(size_t)lb[k / n] * ns + lk[k % n]];
+    for (int k = tid; k < n; k += 64) {
+      prow[k] = k;
+      pcol[k] = k;
     }
     __syncthreads();
-    // Gamma^t accumulates alpha then beta (a docc orbital sits in both lists)
-    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-      const int c = k / n, r = k % n;
-      const size_t pos = (size_t)lb[r] * ns + lk[c];
-      Xt[pos] += aug[(size_t)c * 2 * n + n + r];
+    double sign = 1.0;
+    for (int k = 0; k < n; ++k) {
+      const int m = n - k;
+      double bv = -1.0;
+      int bi = k * n + k;
+      for (int e = tid; e < m * m; e += 64) {
+        const int i = k + e / m, j = k + e % m;
+        const double v = fabs(M[i * n + j]);
+        if (v > bv || (v == bv && i * n + j < bi)) {
+          bv = v;
+          bi = i * n + j;
+        }
+      }
+      wave_argmax(bv, bi);
+      const int pi = bi / n, pj = bi % n;
+      if (pi != k) {
+        for (int j = tid; j < n; j += 64) {
+          const double x = M[k * n + j];
+          M[k * n + j] = M[pi * n + j];
+          M[pi * n + j] = x;
+        }
+        sign = -sign;
+      }
+      __syncthreads();
+      if (pj != k) {
+        for (int i = tid; i < n; i += 64) {
+          const double x = M[i * n + k];
+          M[i * n + k] = M[i * n + pj];
+          M[i * n + pj] = x;
+        }
+        sign = -sign;
+      }
+      if (tid == 0) {
+        int x = prow[k]; prow[k] = prow[pi]; prow[pi] = x;
+        x = pcol[k]; pcol[k] = pcol[pj]; pcol[pj] = x;
+      }
+      __syncthreads();
+      const double d = M[k * n + k];
+      if (tid == 0) piv[k] = d;
+      if (d != 0.0) {  // d == 0: the trailing block is exactly zero, nothing to eliminate
+        for (int i = k + 1 + tid; i < n; i += 64) M[i * n + k] /= d;
+        __syncthreads();
+        const int r = m - 1;
+        for (int e = tid; e < r * r; e += 64) {
+          const int i = k + 1 + e / r, j = k + 1 + e % r;
+          M[i * n + j] -= M[i * n + k] * M[k * n + j];
+        }
+      }
+      __syncthreads();
+    }
+    // unit-triangular inverses, one column per thread (n <= 64)
+    if (tid < n) {
+      const int j = tid;
+      for (int i = 0; i < n; ++i) Ui[i * n + j] = i == j ? 1.0 : 0.0;
+      for (int i = j - 1; i >= 0; --i) {
+        const double di = piv[i];
+        double x = 0.0;
+        if (di != 0.0)
+          for (int mm = i + 1; mm <= j; ++mm) x -= M[i * n + mm] / di * Ui[mm * n + j];
+        Ui[i * n + j] = x;
+      }
+      for (int i = 0; i < n; ++i) Li[i * n + j] = i == j ? 1.0 : 0.0;
+      for (int i = j + 1; i < n; ++i) {
+        double x = 0.0;
+        for (int mm = j; mm < i; ++mm) x -= M[i * n + mm] * Li[mm * n + j];
+        Li[i * n + j] = x;
+      }
+    }
+    __syncthreads();
+    const int z1 = n - 1, z2 = n - 2;
+    const double d1 = piv[z1], d2 = n >= 2 ? piv[z2] : 1.0;
+    double P = 1.0;
+    for (int k = 0; k < n - 2; ++k) P *= piv[k];
+    const double pmax = fabs(piv[0]);
+    if (tid == 0) {
+      q[0] = sign; q[1] = P; q[2] = d1; q[3] = d2;
+      if (!(fabs(d1) > 1e-9 * pmax) || !(fabs(d2) > 1e-9 * pmax)) atomicOr(a.status + 3, 1);
+    }
+    // Gamma_R, g1, g2 at (bra orbital of row prow[i], ket orbital of column pcol[j])
+    for (int e = tid; e < n * n; e += 64) {
+      const int j = e / n, i = e % n;
+      double gr = 0.0;
+      if (P != 0.0)
+        for (int k = 0; k < n - 2; ++k) gr += Ui[j * n + k] * Li[k * n + i] / piv[k];
+      const size_t pos = (size_t)lb[prow[i]] * ns + lk[pcol[j]];
+      Yr[pos] = gr;
+      Y1[pos] = Ui[j * n + z1] * Li[z1 * n + i];
+      if (n >= 2) Y2[pos] = Ui[j * n + z2] * Li[z2 * n + i];
     }
   }
-  if (threadIdx.x == 0) a.w[t] = a.cdet[S] * a.cdet[T] * dets[0] * dets[1];
 }
 
 __device__ __forceinline__ double block_sum64(double v) {
@@ -224,27 +267,129 @@ __device__ __forceinline__ double block_sum64(double v) {
   return v;
 }
 
-// Phi = e1 + e2 per determinant pair (one wavefront each); num = omega * Phi, e1w = omega * e1.
-__global__ void __launch_bounds__(64) k_pair_energy(const double* __restrict__ h, const double* __restrict__ Xa,
-                                                    const double* __restrict__ Xb, const double* __restrict__ Xt,
-                                                    const double* __restrict__ JX, const double* __restrict__ KXa,
-                                                    const double* __restrict__ KXb, const double* __restrict__ w,
-                                                    int n2, long p0, double* __restrict__ phi,
-                                                    double* __restrict__ num, double* __restrict__ e1w) {
+// Per determinant pair, from Y = (Gamma_R, g1, g2) per spin and their Coulomb
+// (JY) and exchange (KY) images:
+//   f = c_S c_T [ sum_s h.C1_s det_s' + 1/2 sum_s T_s det_s' + C1_a . J(C1_b) ],  omega = c_S c_T det_a det_b
+// with C1_s, T_s, det_s the split cofactors above (one wavefront per pair).
+__global__ void __launch_bounds__(64) k_pair_energy(const double* __restrict__ h, const double* __restrict__ Y,
+                                                    const double* __restrict__ JY, const double* __restrict__ KY,
+                                                    const double* __restrict__ q4, const double* __restrict__ cdet,
+                                                    int ndet, int n2, long p0, double* __restrict__ phi,
+                                                    double* __restrict__ num, double* __restrict__ wout,
+                                                    double* __restrict__ e1w) {
   const long t = p0 + blockIdx.x;
-  const size_t o = (size_t)t * n2;
-  double e1 = 0.0, e2 = 0.0;
+  const size_t o = (size_t)t * 6 * n2;
+  // sums: hY[6], B[2][4] = (rr, 1r, 2r, 12), X[3][3] = Y_a(alpha) . JY_b(beta)
+  double acc[23];
+#pragma unroll
+  for (int k = 0; k < 23; ++k) acc[k] = 0.0;
   for (int k = threadIdx.x; k < n2; k += 64) {
-    const double xt = Xt[o + k];
-    e1 += h[k] * xt;
-    e2 += 0.5 * xt * JX[o + k] - 0.5 * (Xa[o + k] * KXa[o + k] + Xb[o + k] * KXb[o + k]);
+    double y[6], jy[6], ky[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      y[m] = Y[o + (size_t)m * n2 + k];
+      jy[m] = JY[o + (size_t)m * n2 + k];
+      ky[m] = KY[o + (size_t)m * n2 + k];
+    }
+    const double hk = h[k];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[m] += hk * y[m];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp) {
+      const int b = 3 * sp;
+      const double gr = jy[b] - ky[b];
+      acc[6 + 4 * sp + 0] += y[b] * gr;
+      acc[6 + 4 * sp + 1] += y[b + 1] * gr;
+      acc[6 + 4 * sp + 2] += y[b + 2] * gr;
+      acc[6 + 4 * sp + 3] += y[b + 1] * (jy[b + 2] - ky[b + 2]);
+    }
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) acc[14 + 3 * aa + bb] += y[aa] * jy[3 + bb];
   }
-  e1 = block_sum64(e1);
-  e2 = block_sum64(e2);
+#pragma unroll
+  for (int k = 0; k < 23; ++k) acc[k] = block_sum64(acc[k]);
   if (threadIdx.x == 0) {
-    phi[t] = e1 + e2;
-    num[t] = w[t] * (e1 + e2);
-    e1w[t] = w[t] * e1;
+    const int S = (int)(t / ndet), T = (int)(t % ndet);
+    const double cst = cdet[S] * cdet[T];
+    double det[2], e1[2], tt[2], cf[2][3];
+    for (int sp = 0; sp < 2; ++sp) {
+      const double* q = q4 + ((size_t)t * 2 + sp) * 4;
+      const double sP = q[0] * q[1], d1 = q[2], d2 = q[3];
+      det[sp] = sP * d1 * d2;
+      cf[sp][0] = sP * d1 * d2;
+      cf[sp][1] = sP * d2;
+      cf[sp][2] = sP * d1;
+      e1[sp] = cf[sp][0] * acc[3 * sp] + cf[sp][1] * acc[3 * sp + 1] + cf[sp][2] * acc[3 * sp + 2];
+      const double* B = acc + 6 + 4 * sp;
+      tt[sp] = sP * (d1 * d2 * B[0] + 2.0 * d2 * B[1] + 2.0 * d1 * B[2] + 2.0 * B[3]);
+    }
+    double cross = 0.0;
+    for (int aa = 0; aa < 3; ++aa)
+      for (int bb = 0; bb < 3; ++bb) cross += cf[0][aa] * cf[1][bb] * acc[14 + 3 * aa + bb];
+    const double one = e1[0] * det[1] + e1[1] * det[0];
+    const double f = cst * (one + 0.5 * (tt[0] * det[1] + tt[1] * det[0]) + cross);
+    const double om = cst * det[0] * det[1];
+    num[t] = f;
+    wout[t] = om;
+    e1w[t] = cst * one;
+    phi[t] = om != 0.0 ? f / om : 0.0;
+  }
+}
+
+// Regular blocks (the gradient path): Gamma_s = Gamma_R + g1/d1 + g2/d2 is the
+// full inverse, with its Coulomb and exchange images by linearity; slot-space
+// inverses for the adjoint.
+__global__ void __launch_bounds__(64) k_pair_regular(const double* __restrict__ Y, const double* __restrict__ JY,
+                                                     const double* __restrict__ KY, const double* __restrict__ q4,
+                                                     const int* __restrict__ alist, const int* __restrict__ blist,
+                                                     int ndet, int na, int nb, int ns, long p0,
+                                                     double* __restrict__ Xa, double* __restrict__ Xb,
+                                                     double* __restrict__ Xt, double* __restrict__ JX,
+                                                     double* __restrict__ KXa, double* __restrict__ KXb,
+                                                     double* __restrict__ Ga, double* __restrict__ Gb) {
+  const long t = p0 + blockIdx.x;
+  const int n2 = ns * ns;
+  const size_t o = (size_t)t * 6 * n2, ot = (size_t)t * n2;
+  double w[2][3];
+  for (int sp = 0; sp < 2; ++sp) {
+    const double* q = q4 + ((size_t)t * 2 + sp) * 4;
+    w[sp][0] = 1.0;
+    w[sp][1] = 1.0 / q[2];
+    w[sp][2] = 1.0 / q[3];
+  }
+  for (int k = threadIdx.x; k < n2; k += 64) {
+    double x[2], jx = 0.0, kx[2];
+    for (int sp = 0; sp < 2; ++sp) {
+      x[sp] = 0.0;
+      kx[sp] = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        const size_t idx = o + (size_t)(3 * sp + m) * n2 + k;
+        x[sp] += w[sp][m] * Y[idx];
+        jx += w[sp][m] * JY[idx];
+        kx[sp] += w[sp][m] * KY[idx];
+      }
+    }
+    Xa[ot + k] = x[0];
+    Xb[ot + k] = x[1];
+    Xt[ot + k] = x[0] + x[1];
+    JX[ot + k] = jx;
+    KXa[ot + k] = kx[0];
+    KXb[ot + k] = kx[1];
+  }
+  __syncthreads();
+  const int S = (int)(t / ndet), T = (int)(t % ndet);
+  for (int sp = 0; sp < 2; ++sp) {
+    const int n = sp == 0 ? na : nb;
+    const int* lb = (sp == 0 ? alist : blist) + (size_t)S * n;
+    const int* lk = (sp == 0 ? alist : blist) + (size_t)T * n;
+    const double* X = sp == 0 ? Xa + ot : Xb + ot;
+    double* G = sp == 0 ? Ga + (size_t)t * na * na : Gb + (size_t)t * nb * nb;
+    for (int k = threadIdx.x; k < n * n; k += 64) {
+      const int c = k / n, r = k % n;
+      G[k] = X[(size_t)lb[r] * ns + lk[c]];
+    }
   }
 }
 
@@ -412,7 +557,7 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
   const int np = s->npair, nnd = 2 * s->npair + s->nunpd;
   sc.na = s->npair + s->nunpd + s->ndocc;
   sc.nb = s->npair + s->ndocc;
-  if (sc.na > kMaxSpinDim) return fail(VSVB_ERR_UNSUPPORTED, "alpha block larger than 60 orbitals");
+  if (sc.na > kMaxSpinDim) return fail(VSVB_ERR_UNSUPPORTED, "alpha block larger than 48 orbitals");
   sc.ndet = s->nspinc << np;
   sc.npairs = (long)sc.ndet * sc.ndet;
   const int ns = c->nsp;
@@ -452,7 +597,7 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
     }
   const size_t n2 = (size_t)ns * ns, nn = (size_t)c->nbf * c->nbf, P = (size_t)sc.npairs;
   const double bytes = 8.0 * (2.0 * sc.npq * nn + (double)sc.npq * c->nbf * ns + 4.0 * n2 * n2 +
-                              P * (sc.na * sc.na + sc.nb * sc.nb + 7.0 * n2 + 4.0) + n2 * ns * sc.npq);
+                              P * (sc.na * sc.na + sc.nb * sc.nb + 25.0 * n2 + 12.0) + n2 * ns * sc.npq);
   if (bytes > 64e9) return fail(VSVB_ERR_UNSUPPORTED, "spin-coupled work arrays would exceed 64 GB");
   HIP_OK(sc.d_alist.alloc(al.size()));
   HIP_OK(sc.d_blist.alloc(bl.size()));
@@ -483,6 +628,10 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
   HIP_OK(sc.d_Dk.alloc(n2 * n2));
   HIP_OK(sc.d_Dh.alloc(n2 * sc.npq));
   HIP_OK(sc.d_Xw.alloc(P * n2));
+  HIP_OK(sc.d_Y.alloc(6 * P * n2));
+  HIP_OK(sc.d_JY.alloc(6 * P * n2));
+  HIP_OK(sc.d_KY.alloc(6 * P * n2));
+  HIP_OK(sc.d_q4.alloc(8 * P));
   HIP_OK(hipMemcpy(sc.d_alist.p, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(sc.d_blist.p, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(sc.d_cdet.p, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -512,25 +661,24 @@ int vsvb::sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev) {
                      sc.d_pq.p, sc.npq, ns, nbf, sc.d_eri.p);
   hipLaunchKernelGGL(k_orb_erik, dim3(grid_for((size_t)n2 * n2)), dim3(256), 0, st, sc.d_eri.p, ns, sc.d_erik.p);
   HIP_OK(hipGetLastError());
-  // determinant pairs: inverses, determinants, transition densities
+  // determinant pairs: split cofactor data (exact for singular blocks)
   DetArgs da{c->d_Sorb.p, sc.d_alist.p, sc.d_blist.p, sc.d_cdet.p, sc.ndet, sc.na, sc.nb, ns,
-             sc.d_Ga.p, sc.d_Gb.p, sc.d_Xa.p, sc.d_Xb.p, sc.d_Xt.p, sc.d_w.p, c->d_status.p};
+             sc.d_Y.p, sc.d_q4.p, c->d_status.p};
   const int dmax = std::max(sc.na, sc.nb);
-  const size_t lds_det = (2 * (size_t)dmax * dmax + dmax) * sizeof(double);
+  const size_t lds_det = 3 * (size_t)dmax * dmax * sizeof(double);
   const long chunk = 1L << 30;
   for (long p0 = 0; p0 < P; p0 += chunk) {
     const unsigned nb = (unsigned)std::min(chunk, P - p0);
     hipLaunchKernelGGL(k_det_pairs, dim3(nb), dim3(64), lds_det, st, da, p0);
     HIP_OK(hipGetLastError());
   }
-  // Coulomb and exchange contractions of every pair: [P x n^2] x [n^2 x n^2]
-  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xt.p, n2, sc.d_eri.p, n2, false, 0.0, sc.d_JX.p, n2));
-  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xa.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KXa.p, n2));
-  HIP_OK(gemm_rows(st, P, n2, n2, 1.0, sc.d_Xb.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KXb.p, n2));
+  // Coulomb and exchange images of the 6 matrices of every pair: [6P x n^2] x [n^2 x n^2]
+  HIP_OK(gemm_rows(st, 6 * P, n2, n2, 1.0, sc.d_Y.p, n2, sc.d_eri.p, n2, false, 0.0, sc.d_JY.p, n2));
+  HIP_OK(gemm_rows(st, 6 * P, n2, n2, 1.0, sc.d_Y.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KY.p, n2));
   for (long p0 = 0; p0 < P; p0 += chunk) {
     const unsigned nb = (unsigned)std::min(chunk, P - p0);
-    hipLaunchKernelGGL(k_pair_energy, dim3(nb), dim3(64), 0, st, c->d_Horb.p, sc.d_Xa.p, sc.d_Xb.p, sc.d_Xt.p,
-                       sc.d_JX.p, sc.d_KXa.p, sc.d_KXb.p, sc.d_w.p, n2, p0, sc.d_phi.p, sc.d_num.p, sc.d_e1w.p);
+    hipLaunchKernelGGL(k_pair_energy, dim3(nb), dim3(64), 0, st, c->d_Horb.p, sc.d_Y.p, sc.d_JY.p, sc.d_KY.p,
+                       sc.d_q4.p, sc.d_cdet.p, sc.ndet, n2, p0, sc.d_phi.p, sc.d_num.p, sc.d_w.p, sc.d_e1w.p);
     HIP_OK(hipGetLastError());
   }
   HIP_OK(launch_reduce(st, sc.d_num.p, P, 1, c->d_scal.p + 0));
@@ -540,7 +688,14 @@ int vsvb::sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev) {
   HIP_OK(hipGetLastError());
   if (!want_grad) return VSVB_OK;
 
-  // ---- adjoint
+  // ---- adjoint (every block regular: checked at the end of the call)
+  for (long p0 = 0; p0 < P; p0 += chunk) {
+    const unsigned nb = (unsigned)std::min(chunk, P - p0);
+    hipLaunchKernelGGL(k_pair_regular, dim3(nb), dim3(64), 0, st, sc.d_Y.p, sc.d_JY.p, sc.d_KY.p, sc.d_q4.p,
+                       sc.d_alist.p, sc.d_blist.p, sc.ndet, sc.na, sc.nb, ns, p0, sc.d_Xa.p, sc.d_Xb.p, sc.d_Xt.p,
+                       sc.d_JX.p, sc.d_KXa.p, sc.d_KXb.p, sc.d_Ga.p, sc.d_Gb.p);
+    HIP_OK(hipGetLastError());
+  }
   HIP_OK(launch_fill(st, sc.d_Shat.p, n2, 0.0));
   AdjArgs aa{c->d_Horb.p, sc.d_JX.p, sc.d_KXa.p, sc.d_KXb.p, sc.d_Ga.p, sc.d_Gb.p, sc.d_w.p, sc.d_phi.p,
              c->d_scal.p, sc.d_alist.p, sc.d_blist.p, sc.ndet, sc.na, sc.nb, ns, sc.d_Shat.p, sc.d_wv.p};
@@ -584,6 +739,13 @@ int vsvb::sc_finish(vsvb_context* c, double* energy, double* parts, double* grad
   if (grad_host)
     HIP_OK(hipMemcpyAsync(grad_host, c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if ((rc = check_status(c, grad_host != nullptr))) return rc;
+  if (grad_host) {
+    int st3 = 0;
+    HIP_OK(hipMemcpy(&st3, c->d_status.p + 3, sizeof(int), hipMemcpyDeviceToHost));
+    if (st3)
+      return fail(VSVB_ERR_UNSUPPORTED, "gradient at a singular or nearly singular transition overlap of the "
+                                        "spin-coupled expansion (pivot ratio below 1e-9); the energy is exact there");
+  }
   if ((rc = finish_counters(c))) return rc;
   if (!std::isfinite(h[2])) return fail(VSVB_ERR_NUMERIC, "non-finite energy");
   if (energy) *energy = h[2];
