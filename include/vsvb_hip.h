@@ -57,7 +57,7 @@ typedef struct vsvb_system {
     const int32_t *orb_start;           /* [norbs+1]          map_orbs - 1        */
     const int32_t *xpset;               /* [totlen]           xpset (1-based; <1 = NDF) */
     const double *coeff;                /* [totlen]           coeff               */
-    const int32_t *pair_sc;             /* [max(nspinc,1)][npair][2] pair_sc (1-based slots) */
+    const int32_t *pair_sc;             /* [max(nspinc,1)][2][npair] pair_sc(npair,2,nspinc) as stored (1-based slots) */
     const double *coeff_sc;             /* [max(nspinc,1)]    coeff_sc            */
 } vsvb_system;
 

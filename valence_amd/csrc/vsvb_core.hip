@@ -599,26 +599,27 @@ static int pair_class_of(int la, int lb) {
   return -1;
 }
 
-static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st) {
+static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds,
+                                       hipStream_t st) {
   switch (bc) {
-    case 0: return launch_quartet_b0(kc, a, jk, nblocks, st);
-    case 1: return launch_quartet_b1(kc, a, jk, nblocks, st);
-    case 2: return launch_quartet_b2(kc, a, jk, nblocks, st);
-    case 3: return launch_quartet_b3(kc, a, jk, nblocks, st);
-    case 4: return launch_quartet_b4(kc, a, jk, nblocks, st);
-    case 5: return launch_quartet_b5(kc, a, jk, nblocks, st);
+    case 0: return launch_quartet_b0(kc, a, jk, nblocks, lds, st);
+    case 1: return launch_quartet_b1(kc, a, jk, nblocks, lds, st);
+    case 2: return launch_quartet_b2(kc, a, jk, nblocks, lds, st);
+    case 3: return launch_quartet_b3(kc, a, jk, nblocks, lds, st);
+    case 4: return launch_quartet_b4(kc, a, jk, nblocks, lds, st);
+    case 5: return launch_quartet_b5(kc, a, jk, nblocks, lds, st);
   }
   return hipErrorInvalidValue;
 }
 
-static int class_occupancy(int bc, int kc, bool jk) {
+static int class_occupancy(int bc, int kc, bool jk, size_t lds) {
   switch (bc) {
-    case 0: return occupancy_b0(kc, jk);
-    case 1: return occupancy_b1(kc, jk);
-    case 2: return occupancy_b2(kc, jk);
-    case 3: return occupancy_b3(kc, jk);
-    case 4: return occupancy_b4(kc, jk);
-    case 5: return occupancy_b5(kc, jk);
+    case 0: return occupancy_b0(kc, jk, lds);
+    case 1: return occupancy_b1(kc, jk, lds);
+    case 2: return occupancy_b2(kc, jk, lds);
+    case 3: return occupancy_b3(kc, jk, lds);
+    case 4: return occupancy_b4(kc, jk, lds);
+    case 5: return occupancy_b5(kc, jk, lds);
   }
   return 1;
 }
@@ -822,11 +823,16 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       const int q = quartet_class_index(bc, kc);
       const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
       const int tb = (nb + kTileB - 1) / kTileB;
-      c->ntiles_class[q] = (nb > 0 && nk > 0) ? tb * nk : 0;  // work units, one wavefront each
+      c->ntiles_class[q] = (nb > 0 && nk > 0) ? tb * nk : 0;  // energy work units, one wavefront each
       const int waves = (c->ntiles_class[q] + 3) / 4;
       // Low-AM energy kernels measured fastest with at most 2048 workgroups (8192 wavefronts).
-      c->blocks_per_class[q] = std::min(waves, std::min(2048, class_occupancy(bc, kc, false) * ncu));
-      c->blocks_jk[q] = std::min(waves, class_occupancy(bc, kc, true) * ncu);
+      c->blocks_per_class[q] = std::min(waves, std::min(2048, class_occupancy(bc, kc, false, 0) * ncu));
+      // J/K: one workgroup per (256-pair bra strip, ket pair); exchange rows in LDS when they fit
+      const int nstrip = (nb + kStripB - 1) / kStripB;
+      c->ntiles_jk[q] = (nb > 0 && nk > 0) ? nstrip * nk : 0;
+      const size_t kb = (size_t)c->nbf * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) * sizeof(double);
+      c->lds_jk[q] = kb <= kMaxLdsK ? kb : 0;
+      c->blocks_jk[q] = std::min(c->ntiles_jk[q], class_occupancy(bc, kc, true, c->lds_jk[q]) * ncu);
       c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;  // energy mode writes every entry
       c->partial_total += c->partial_per_class[q];
     }
@@ -979,7 +985,8 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       a.nket = c->pair_off[kc + 1] - a.ket0;
       a.same = bc == kc;
       a.ntk = (a.nket + kTileK - 1) / kTileK;
-      a.ntiles = c->ntiles_class[q];
+      a.ntiles = jk ? c->ntiles_jk[q] : c->ntiles_class[q];
+      a.lds_k = jk && c->lds_jk[q] > 0;
       a.tau = c->tau;
       a.nbf = c->nbf;
       a.nwfn = nwfn;
@@ -991,7 +998,8 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       a.counters = c->d_counters.p + 4 * q;
       a.tile_ctr = c->d_tilectr.p + q;
       if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
-      HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q], st));
+      HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q],
+                                  jk ? c->lds_jk[q] : 0, st));
       poff += c->partial_per_class[q];
     }
   if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));
@@ -1030,14 +1038,15 @@ int vsvb::finish_counters(vsvb_context* c) {
   for (int q = 0; q < kNQuartetClass; ++q) {
     for (int k = 0; k < 3; ++k) tot[k] += (double)h[4 * q + k];
     float cms = 0.0f;
-    if (c->profile && c->ntiles_class[q] > 0) HIP_OK(hipEventElapsedTime(&cms, c->cls_ev[q], c->cls_ev[q + 1]));
+    if (c->profile && (c->last_jk ? c->ntiles_jk[q] : c->ntiles_class[q]) > 0)
+      HIP_OK(hipEventElapsedTime(&cms, c->cls_ev[q], c->cls_ev[q + 1]));
     c->class_stats[q][0] = cms;
     c->class_stats[q][1] = (double)h[4 * q];
     c->class_stats[q][2] = (double)h[4 * q + 1];
     c->class_stats[q][3] = (double)h[4 * q + 2];
     c->class_stats[q][4] = (double)h[4 * q + 1] * kFlopsPerPrimQuartet[q] +
                            (double)h[4 * q] * (kFlopsHrr[q] + c->last_nwfn * digestion_flops(q, nspin, c->last_jk));
-    c->class_stats[q][5] = c->ntiles_class[q] > 0 ? 1.0 : 0.0;
+    c->class_stats[q][5] = (c->last_jk ? c->ntiles_jk[q] : c->ntiles_class[q]) > 0 ? 1.0 : 0.0;
   }
   c->last_counters[0] = tot[0];
   c->last_counters[1] = tot[1];

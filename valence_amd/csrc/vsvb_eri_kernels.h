@@ -38,7 +38,7 @@ struct QuartetArgs {
   int ket0, nket;       // pair index range of the ket class
   int same;             // bra class == ket class: only ket <= bra
   int ntk;              // unused
-  int ntiles;           // work units: bra tiles (64 pairs) * ket pairs
+  int ntiles;           // work units: energy: bra tiles (64 pairs) x ket pairs; J/K: bra strips (256) x ket pairs
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;
   int nwfn;             // wave functions sharing this geometry
@@ -50,6 +50,7 @@ struct QuartetArgs {
   double* kacc;         // J/K mode: exchange accumulators [nwfn][nspin][nbf][nbf]
   unsigned long long* counters;  // [0] quartets evaluated, [1] primitive quartets, [2] skipped
   int* tile_ctr;        // dynamic tile counter (zero before the launch)
+  int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
 };
 
 // A work unit is 64 bra pairs x 1 ket pair for one wavefront: the ket pair is
@@ -87,8 +88,8 @@ struct DBlocks {
   }
 };
 
-template <int LA, int LB, int LC, int LD, bool kJK>
-__global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
   const int n = a.nbf;
@@ -161,112 +162,36 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
           if (a.nspin == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
           rcd[k * ND + l] = cfac * t;
         }
-      if constexpr (!kJK) {
-        double ej = 0.0;
+      double ej = 0.0;
 #pragma unroll
-        for (int ij = 0; ij < NAB; ++ij) {
-          double t = 0.0;
+      for (int ij = 0; ij < NAB; ++ij) {
+        double t = 0.0;
 #pragma unroll
-          for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-          ej += rab[ij] * t;
-        }
-        double ek = 0.0;
-        for (int s = 0; s < a.nspin; ++s) {
-          DBlocks<NA, NB, NC, ND> d;
-          d.load(D0 + s * nn, n, ma, mb, mc, md);
-#pragma unroll
-          for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
-#pragma unroll
-              for (int k = 0; k < NC; ++k)
-#pragma unroll
-                for (int l = 0; l < ND; ++l)
-                  ek += v[((i * NB + j) * NC + k) * ND + l] *
-                        (d.ac[i * NC + k] * d.bd[j * ND + l] + d.ad[i * ND + l] * d.bc[j * NC + k]);
-        }
-        const double e = deg * (0.5 * ej - 0.25 * xfac * ek);
-#pragma unroll
-        for (int ww = 0; ww < kMaxW; ++ww)
-          if (ww == w) esum[ww] += e;
-      } else {
-        // Coulomb: A_mn += deg/4 sum v RT_ls ; A_ls += deg/4 sum v RT_mn      (J = A + A^T)
-        double* J = a.jacc + (size_t)w * nn;
-        const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
-        if (act) {
-#pragma unroll
-          for (int ij = 0; ij < NAB; ++ij) {
-            double t = 0.0;
-#pragma unroll
-            for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-            unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
-          }
-        }
-        // ket block is shared by the wave: reduce over lanes, one atomic per element
-#pragma unroll
-        for (int kl = 0; kl < NCD; ++kl) {
-          double t = 0.0;
-#pragma unroll
-          for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
-          t = wave_sum(fj * t);
-          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], t);
-        }
-        // Exchange (carries x_s): B_ml += v D_ns ; B_nl += v D_ms ; B_ms += v D_nl ; B_ns += v D_ml  (K = B + B^T)
-        if (act) {
-          for (int s = 0; s < a.nspin; ++s) {
-            DBlocks<NA, NB, NC, ND> d;
-            d.load(D0 + s * nn, n, ma, mb, mc, md);
-            double* Kx = a.kacc + ((size_t)w * a.nspin + s) * nn;
-#pragma unroll
-            for (int i = 0; i < NA; ++i)
-#pragma unroll
-              for (int k = 0; k < NC; ++k) {
-                double t = 0.0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
-                unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
-              }
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
-#pragma unroll
-              for (int k = 0; k < NC; ++k) {
-                double t = 0.0;
-#pragma unroll
-                for (int i = 0; i < NA; ++i)
-#pragma unroll
-                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
-                unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
-              }
-#pragma unroll
-            for (int i = 0; i < NA; ++i)
-#pragma unroll
-              for (int l = 0; l < ND; ++l) {
-                double t = 0.0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
-                unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
-              }
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
-#pragma unroll
-              for (int l = 0; l < ND; ++l) {
-                double t = 0.0;
-#pragma unroll
-                for (int i = 0; i < NA; ++i)
-#pragma unroll
-                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
-                unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
-              }
-          }
-        }
+        for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+        ej += rab[ij] * t;
       }
+      double ek = 0.0;
+      for (int s = 0; s < a.nspin; ++s) {
+        DBlocks<NA, NB, NC, ND> d;
+        d.load(D0 + s * nn, n, ma, mb, mc, md);
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+#pragma unroll
+              for (int l = 0; l < ND; ++l)
+                ek += v[((i * NB + j) * NC + k) * ND + l] *
+                      (d.ac[i * NC + k] * d.bd[j * ND + l] + d.ad[i * ND + l] * d.bc[j * NC + k]);
+      }
+      const double e = deg * (0.5 * ej - 0.25 * xfac * ek);
+#pragma unroll
+      for (int ww = 0; ww < kMaxW; ++ww)
+        if (ww == w) esum[ww] += e;
     }
   }
-  if constexpr (!kJK) {
+  {
     // deterministic: shuffle-reduce each wave, one partial per (wave, wave function)
     const int nw = a.nwfn < kMaxW ? a.nwfn : kMaxW;
     for (int w = 0; w < nw; ++w) {
@@ -291,6 +216,208 @@ __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
       atomicAdd(&a.counters[2], sk);
     }
   }
+}
+
+
+// J/K (adjoint) contraction.  A workgroup takes one ket pair and a strip of
+// four 64-pair bra tiles (one per wavefront) from a counter.  Coulomb:
+// A_mn += deg/4 sum v RT_ls per lane and A_ls += deg/4 sum v RT_mn reduced over
+// the wave (J = A + A^T).  Exchange (carries x_s): B_ml += v D_ns, B_nl += v D_ms,
+// B_ms += v D_nl, B_ns += v D_ml (K = B + B^T): every update lands in a row of
+// the bra AOs and in one of the NC + ND columns of the unit's ket AOs, so with
+// lds_k the unit accumulates them in an nbf x (NC + ND) LDS block (LDS atomics)
+// and writes each nonzero entry to HBM once per unit, instead of one global
+// atomic per quartet element; without lds_k (block too large) they go straight
+// to the global accumulators.
+constexpr int kStripB = 4 * kTileB;
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NK = NC + ND;
+  extern __shared__ double kbuf[];
+  __shared__ int s_unit;
+  const int n = a.nbf;
+  const size_t nn = (size_t)n * n;
+  const int ns = a.nspin;
+  const int dstride = ns > 0 ? ns : 1;
+  const double cfac = ns == 1 ? 2.0 : 1.0;
+  const double xfac = ns == 1 ? 2.0 : 1.0;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool lds = a.lds_k != 0;
+  const int kb_len = lds ? n * NK : 0;
+  for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
+  unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  for (;;) {
+    __syncthreads();  // the previous unit is flushed and s_unit is free
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr, 1);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (u >= a.ntiles) break;
+    const int st = u / a.nket;
+    const int ik = __builtin_amdgcn_readfirstlane(u - st * a.nket);
+    if (a.same && ik > st * kStripB + kStripB - 1) continue;  // the whole strip is above the diagonal
+    const int ib = st * kStripB + wave * kTileB + lane;
+    const Pair K = a.pairs[a.ket0 + ik];
+    bool act = ib < a.nbra && (!a.same || ik <= ib);
+    Pair P{};
+    if (act) {
+      P = a.pairs[a.bra0 + ib];
+      if (P.Q * K.Q < a.tau) {
+        act = false;
+        ++nskip;
+      }
+    }
+    double v[NAB * NCD];
+    if (act) {
+      eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                K.ABz, a.boys, v);
+      ++nq;
+      nprimq += (unsigned long long)P.npp * K.npp;
+    } else {
+#pragma unroll
+      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
+    }
+    const bool wact = __any(act);  // wave-uniform
+    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
+                           : 0.0;
+    // inactive lanes read a valid (ket) block and contribute zero
+    const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
+    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+    const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+    for (int w = 0; w < a.nwfn; ++w) {
+      const double* D0 = a.dens + (size_t)w * dstride * nn;
+      if (wact) {
+        double rab[NAB], rcd[NCD];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            double t = D0[(size_t)(ma + i) * n + mb + j];
+            if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
+            rab[i * NB + j] = cfac * t;
+          }
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) {
+            double t = D0[(size_t)(mc + k) * n + md + l];
+            if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
+            rcd[k * ND + l] = cfac * t;
+          }
+        double* J = a.jacc + (size_t)w * nn;
+        if (act) {
+#pragma unroll
+          for (int ij = 0; ij < NAB; ++ij) {
+            double t = 0.0;
+#pragma unroll
+            for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+            unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
+          }
+        }
+#pragma unroll
+        for (int kl = 0; kl < NCD; ++kl) {
+          double t = 0.0;
+#pragma unroll
+          for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
+          t = wave_sum(fj * t);
+          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], t);
+        }
+      }
+      for (int s = 0; s < ns; ++s) {
+        double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+        if (act) {
+          DBlocks<NA, NB, NC, ND> d;
+          d.load(D0 + s * nn, n, ma, mb, mc, md);
+          // target (row, ket column c < NK): LDS block or the global accumulator
+          auto put = [&](int row, int c, double x) {
+            if (lds)
+              unsafeAtomicAdd(&kbuf[row * NK + c], x);
+            else
+              unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
+          };
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+              double t = 0.0;
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
+              put(ma + i, k, fk * t);
+            }
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
+              put(mb + j, k, fk * t);
+            }
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+              double t = 0.0;
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
+              put(ma + i, NC + l, fk * t);
+            }
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
+              put(mb + j, NC + l, fk * t);
+            }
+        }
+        if (lds) {
+          __syncthreads();
+          for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
+            const double x = kbuf[e];
+            if (x != 0.0) {
+              kbuf[e] = 0.0;
+              const int row = e / NK, c = e - row * NK;
+              unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (a.counters) {
+    unsigned long long q = nq, pq = nprimq, sk = nskip;
+    for (int o = 32; o > 0; o >>= 1) {
+      q += __shfl_xor(q, o, 64);
+      pq += __shfl_xor(pq, o, 64);
+      sk += __shfl_xor(sk, o, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], q);
+      atomicAdd(&a.counters[1], pq);
+      atomicAdd(&a.counters[2], sk);
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD, bool kJK>
+__global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
+  if constexpr (kJK)
+    quartet_jk_body<LA, LB, LC, LD>(a);
+  else
+    quartet_energy_body<LA, LB, LC, LD>(a);
 }
 
 // Schwarz bound of every pair of one class, Q = sqrt(max_ab |(ab|ab)|): one
@@ -321,38 +448,38 @@ __global__ void __launch_bounds__(256) schwarz_kernel(Pair* pairs, const PrimPai
 }
 
 template <int LA, int LB, int LC, int LD>
-static int occupancy_q(bool jk) {
+static int occupancy_q(bool jk, size_t lds_bytes) {
   int n = 0;
   if (jk)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, true>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, true>, 256, lds_bytes);
   else
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, false>, 256, 0);
   return n > 0 ? n : 1;
 }
 
 template <int LA, int LB, int LC, int LD>
-static hipError_t launch_q(const QuartetArgs& a, bool jk, int nblocks, hipStream_t st) {
+static hipError_t launch_q(const QuartetArgs& a, bool jk, int nblocks, size_t lds_bytes, hipStream_t st) {
   if (nblocks <= 0 || a.ntiles <= 0) return hipSuccess;
   if (jk)
-    hipLaunchKernelGGL((quartet_kernel<LA, LB, LC, LD, true>), dim3(nblocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((quartet_kernel<LA, LB, LC, LD, true>), dim3(nblocks), dim3(256), lds_bytes, st, a);
   else
     hipLaunchKernelGGL((quartet_kernel<LA, LB, LC, LD, false>), dim3(nblocks), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 // Per-bra-class launchers (one translation unit each).
-hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-hipError_t launch_quartet_b1(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks, hipStream_t st);
-int occupancy_b0(int kc, bool jk);
-int occupancy_b1(int kc, bool jk);
-int occupancy_b2(int kc, bool jk);
-int occupancy_b3(int kc, bool jk);
-int occupancy_b4(int kc, bool jk);
-int occupancy_b5(int kc, bool jk);
+hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b1(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
+int occupancy_b0(int kc, bool jk, size_t lds);
+int occupancy_b1(int kc, bool jk, size_t lds);
+int occupancy_b2(int kc, bool jk, size_t lds);
+int occupancy_b3(int kc, bool jk, size_t lds);
+int occupancy_b4(int kc, bool jk, size_t lds);
+int occupancy_b5(int kc, bool jk, size_t lds);
 hipError_t launch_schwarz_b0(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b1(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b2(Pair*, const PrimPair*, const double*, int, int, hipStream_t);

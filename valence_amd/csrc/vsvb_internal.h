@@ -15,6 +15,9 @@
 
 namespace vsvb {
 
+// Largest exchange LDS block of a J/K workgroup (nbf x ket columns doubles).
+constexpr size_t kMaxLdsK = 60 * 1024;
+
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
 
@@ -125,6 +128,8 @@ struct vsvb_context {
   size_t partial_per_class[vsvb::kNQuartetClass] = {0};
   int blocks_per_class[vsvb::kNQuartetClass] = {0};
   int ntiles_class[vsvb::kNQuartetClass] = {0};
+  int ntiles_jk[vsvb::kNQuartetClass] = {0};      // J/K work units (256-pair bra strips x ket pairs)
+  size_t lds_jk[vsvb::kNQuartetClass] = {0};     // J/K exchange LDS block per workgroup (0: global atomics)
   size_t partial_total = 0;
   bool geom_ready = false;
   double enuc = 0.0;

@@ -568,7 +568,7 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
     std::vector<int> seen(nnd, 0);
     for (int k = 0; k < np; ++k)
       for (int ab = 0; ab < 2; ++ab) {
-        const int slot = s->pair_sc[((size_t)isc * np + k) * 2 + ab];
+        const int slot = s->pair_sc[((size_t)isc * 2 + ab) * np + k];  // pair_sc(k, ab, isc)
         if (slot < 1 || slot > 2 * np) return fail(VSVB_ERR_ARG, "pair_sc slot outside the spin-coupled orbitals");
         if (seen[slot - 1]++) return fail(VSVB_ERR_ARG, "pair_sc uses an orbital twice in one spin coupling");
       }
@@ -577,7 +577,7 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
       int* A = al.data() + (size_t)d * sc.na;
       int* B = bl.data() + (size_t)d * sc.nb;
       for (int k = 0; k < np; ++k) {
-        int x = s->pair_sc[((size_t)isc * np + k) * 2] - 1, y = s->pair_sc[((size_t)isc * np + k) * 2 + 1] - 1;
+        int x = s->pair_sc[((size_t)isc * 2) * np + k] - 1, y = s->pair_sc[((size_t)isc * 2 + 1) * np + k] - 1;
         if (mask >> k & 1) std::swap(x, y);
         A[k] = x;
         B[k] = y;

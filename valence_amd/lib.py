@@ -106,6 +106,8 @@ def device_boys(L: int, T) -> np.ndarray:
 class _Packed:
     def __init__(self, s: VsvbSystem):
         pk = pack(s)
+        # the ABI takes pair_sc in the reference's storage order, pair_sc(npair, 2, nspinc)
+        pk.i32["pair_sc"] = np.ascontiguousarray(np.asarray(pk.i32["pair_sc"], np.int32).transpose(0, 2, 1))
         self.keep = {**pk.i32, **pk.f64}
         sc = pk.scalars
         self.cs = CSystem(**{k: sc[k] for k in ("natom", "ntype", "nshell_type", "nprim", "norbs", "mxctr", "totlen",
