@@ -96,8 +96,13 @@ int vsvb_energy_parts(vsvb_context *ctx, const double *coeff, double *energy_num
  * grad has totlen entries; d E / d coeff[k] at the given (unnormalized) weights. */
 int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coeff, double *energy, double *grad);
 
-/* Same as vsvb_energy_grad with device pointers on a caller stream
- * (hipStream_t passed as void*).  coeff/energy/grad are device memory. */
+/* Same as vsvb_energy_grad with device pointers (no host staging of the
+ * weights or results): coeff_device [totlen], energy_device [1] and grad_device
+ * [totlen] are device memory of the context's GPU; stream is a hipStream_t
+ * passed as void* (NULL = the null stream).  The evaluation is ordered after the
+ * work already queued on that stream, and work queued there afterwards waits for
+ * the results; the call returns once they are written (so that numerical
+ * errors can be reported).  Geometry: the context's current coordinates. */
 int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
                             double *grad_device, void *stream);
 

@@ -281,12 +281,13 @@ def gen_class(la, lb, lc, ld):
     accmap, nvrr, targets = gen_vrr(la, lb, lc, ld, body)
     acc = {t: f"acc{n}" for n, t in enumerate(targets)}
     em.emit("    " + " ".join(f"double {acc[t]} = 0.0;" for t in targets))
+    em.emit("    const UniformPrimPairs kc = as_uniform(kp);  // the ket list is the same for the whole wavefront")
     em.emit("    for (int ib = 0; ib < nbp; ++ib) {")
     em.emit("      const PrimPair B = bp[ib];")
     if la + lb > 0:
         em.emit("      const double PAx = B.PAx, PAy = B.PAy, PAz = B.PAz, oo2p = B.oo2p;")
     em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
-    em.emit("        const PrimPair K = kp[ik];")
+    em.emit("        const PrimPair K = load_uniform(kc + ik);")
     em.emit("        const double s = B.p + K.p;")
     em.emit("        const double r = rsqrt_fp64(s), rs = r * r;")
     em.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")

@@ -97,15 +97,13 @@ static std::vector<double> make_boys_table() {
 // device: geometry preparation
 // ---------------------------------------------------------------------------
 
-__global__ void k_pair_prep(const PPSrc* __restrict__ src, int npp, const Pair* __restrict__ pairs,
-                            const Shell* __restrict__ shells, const double* __restrict__ pexp,
-                            const double* __restrict__ pcoef, const double* __restrict__ xyz, double kfold,
-                            PrimPair* __restrict__ out) {
+__global__ void k_pair_prep(const PPSrc* __restrict__ src, int npp, const Shell* __restrict__ shells,
+                            const double* __restrict__ pexp, const double* __restrict__ pcoef,
+                            const double* __restrict__ xyz, double kfold, PrimPair* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npp) return;
   const PPSrc s = src[t];
-  const Pair P = pairs[s.pair];
-  const Shell A = shells[P.A], B = shells[P.B];
+  const Shell A = shells[s.sA], B = shells[s.sB];
   const double Ax = xyz[3 * A.atom], Ay = xyz[3 * A.atom + 1], Az = xyz[3 * A.atom + 2];
   const double Bx = xyz[3 * B.atom], By = xyz[3 * B.atom + 1], Bz = xyz[3 * B.atom + 2];
   const double a = pexp[s.ia], b = pexp[s.ib], p = a + b, rp = 1.0 / p;
@@ -648,6 +646,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&c->ev0));
   HIP_OK(hipEventCreate(&c->ev1));
+  HIP_OK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
   c->natom = s->natom;
   c->npair = s->npair;
   c->nunpd = s->nunpd;
@@ -729,13 +729,17 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
     const Shell& A = shells[pairs[i].A];
     const Shell& B = shells[pairs[i].B];
     for (int ia = 0; ia < A.nprim; ++ia)
-      for (int ib = 0; ib < B.nprim; ++ib) ppsrc.push_back(PPSrc{(int)i, A.prim0 + ia, B.prim0 + ib, 0});
+      for (int ib = 0; ib < B.nprim; ++ib) ppsrc.push_back(PPSrc{pairs[i].A, pairs[i].B, A.prim0 + ia, B.prim0 + ib});
   }
   for (int cl = 0; cl <= kNPairClass; ++cl) c->pair_off[cl] = 0;
   for (auto& h : hp) c->pair_off[h.cls + 1]++;
   for (int cl = 0; cl < kNPairClass; ++cl) c->pair_off[cl + 1] += c->pair_off[cl];
   c->npairs = (int)pairs.size();
   c->npp = (int)ppsrc.size();
+  c->h_shells = shells;
+  c->h_pairs = pairs;
+  c->h_pexp = pexp;
+  c->h_pcoef = pcoef;
   // orbital entries: map each weight to an AO of the molecule (OBS-local numbering, xm_input)
   c->ent.resize(std::max(1, s->totlen));
   c->orb_start.assign(s->orb_start, s->orb_start + s->norbs + 1);
@@ -856,17 +860,66 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
 }
 
 // geometry-dependent device state
+// Work order of the shell pairs at one geometry: inside each class, by the
+// number of primitive pairs that survive the primitive screen (descending),
+// then by centre distance, so that the 64 lanes of a wavefront run primitive
+// loops of nearly equal length and Schwarz-skipped quartets cluster.  The
+// order is a schedule only (every unique quartet is still evaluated once); it
+// is recomputed when the coordinates change.
+static int schedule_pairs(vsvb_context* c) {
+  if (c->sched_coords == c->coords) return VSVB_OK;
+  const double kfold = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+  const int np = c->npairs;
+  std::vector<int> kept(np);
+  std::vector<double> r2(np);
+  for (int i = 0; i < np; ++i) {
+    const Pair& P = c->h_pairs[i];
+    const Shell& A = c->h_shells[P.A];
+    const Shell& B = c->h_shells[P.B];
+    const double* xa = c->coords.data() + 3 * A.atom;
+    const double* xb = c->coords.data() + 3 * B.atom;
+    const double d2 = (xa[0] - xb[0]) * (xa[0] - xb[0]) + (xa[1] - xb[1]) * (xa[1] - xb[1]) +
+                      (xa[2] - xb[2]) * (xa[2] - xb[2]);
+    int k = 0;
+    for (int ia = 0; ia < A.nprim; ++ia)
+      for (int ib = 0; ib < B.nprim; ++ib) {
+        const double a = c->h_pexp[A.prim0 + ia], b = c->h_pexp[B.prim0 + ib], p = a + b;
+        const double K = c->h_pcoef[A.prim0 + ia] * c->h_pcoef[B.prim0 + ib] * std::exp(-a * b / p * d2) / p * kfold;
+        if (std::fabs(K) > kPrimCut) ++k;
+      }
+    kept[i] = k;
+    r2[i] = d2;
+  }
+  std::vector<int> idx(np);
+  std::vector<Pair> sorted(np);
+  for (int cl = 0; cl < kNPairClass; ++cl) {
+    const int b = c->pair_off[cl], e = c->pair_off[cl + 1];
+    for (int i = b; i < e; ++i) idx[i] = i;
+    std::stable_sort(idx.begin() + b, idx.begin() + e, [&](int x, int y) {
+      if (kept[x] != kept[y]) return kept[x] > kept[y];
+      return r2[x] < r2[y];
+    });
+  }
+  for (int i = 0; i < np; ++i) sorted[i] = c->h_pairs[idx[i]];
+  c->h_pairs.swap(sorted);
+  HIP_OK(hipMemcpyAsync(c->d_pairs.p, c->h_pairs.data(), np * sizeof(Pair), hipMemcpyHostToDevice, c->stream));
+  c->sched_coords = c->coords;
+  return VSVB_OK;
+}
+
 int vsvb::prepare_geometry(vsvb_context* c, const double* coords) {
   if (coords) c->coords.assign(coords, coords + 3 * c->natom);
   if (c->geom_ready && !coords) return VSVB_OK;
   hipStream_t st = c->stream;
+  int rc;
+  if ((rc = schedule_pairs(c))) return rc;
   c->enuc = nuclear_repulsion(c->natom, c->coords.data(), c->charges.data());
   HIP_OK(hipMemcpyAsync(c->d_xyz.p, c->coords.data(), 3 * c->natom * sizeof(double), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_pair_geom, dim3((c->npairs + 255) / 256), dim3(256), 0, st, c->d_pairs.p, c->npairs,
                      c->d_shells.p, c->d_xyz.p);
   HIP_OK(hipGetLastError());
   const double kfold = std::sqrt(2.0 * std::pow(M_PI, 2.5));
-  hipLaunchKernelGGL(k_pair_prep, dim3((c->npp + 255) / 256), dim3(256), 0, st, c->d_ppsrc.p, c->npp, c->d_pairs.p,
+  hipLaunchKernelGGL(k_pair_prep, dim3((c->npp + 255) / 256), dim3(256), 0, st, c->d_ppsrc.p, c->npp,
                      c->d_shells.p, c->d_pexp.p, c->d_pcoef.p, c->d_xyz.p, kfold, c->d_pp.p);
   HIP_OK(hipGetLastError());
   hipLaunchKernelGGL(k_prim_screen, dim3((c->npairs + 255) / 256), dim3(256), 0, st, c->d_pairs.p, c->npairs,
@@ -1288,6 +1341,8 @@ void vsvb_context_destroy(vsvb_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   for (int q = 0; q <= kNQuartetClass; ++q)
     if (c->cls_ev[q]) (void)hipEventDestroy(c->cls_ev[q]);
   hipStream_t st = c->own_stream ? c->stream : nullptr;
@@ -1328,8 +1383,38 @@ int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff,
 
 int vsvb_energy_grad_device(vsvb_context* c, const double* coeff_device, double* energy_device, double* grad_device,
                             void* stream) {
-  (void)c; (void)coeff_device; (void)energy_device; (void)grad_device; (void)stream;
-  return fail(VSVB_ERR_UNSUPPORTED, "device-pointer gradient not available yet");
+  if (!c || !coeff_device || !energy_device || !grad_device) return fail(VSVB_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  for (int i = 0; i < c->totlen; ++i)
+    if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
+  hipStream_t caller = (hipStream_t)stream;
+  // the caller's earlier work (that produced coeff) -> the context stream -> the caller's later work
+  HIP_OK(hipEventRecord(c->ev_in, caller));
+  HIP_OK(hipStreamWaitEvent(c->stream, c->ev_in, 0));
+  int rc;
+  if ((rc = prepare_geometry(c, nullptr))) return rc;
+  if (c->npair > 0) {
+    if ((rc = prepare_orbital_basis(c, nullptr, coeff_device))) return rc;
+    if ((rc = sc_enqueue(c, true, grad_device))) return rc;
+    HIP_OK(hipMemcpyAsync(energy_device, c->d_scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    if ((rc = prepare_orbitals(c, nullptr, coeff_device))) return rc;
+    if ((rc = run_quartets(c, true, nullptr))) return rc;
+    if ((rc = gradient_impl(c, grad_device))) return rc;
+    hipLaunchKernelGGL(k_final_energy, dim3(1), dim3(1), 0, c->stream, c->d_scal.p, c->d_scal.p + 3, c->enuc,
+                       energy_device);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipEventRecord(c->ev_out, c->stream));
+  HIP_OK(hipStreamWaitEvent(caller, c->ev_out, 0));
+  // numerical status (singular blocks, masked shells) is known only after the work ran
+  if ((rc = check_status(c, true))) return rc;
+  if (c->npair > 0) {
+    int st3 = 0;
+    HIP_OK(hipMemcpy(&st3, c->d_status.p + 3, sizeof(int), hipMemcpyDeviceToHost));
+    if (st3) return fail(VSVB_ERR_UNSUPPORTED, "gradient at a singular spin-coupled transition overlap");
+  }
+  return finish_counters(c);
 }
 
 int vsvb_debug_buffer(vsvb_context* c, const char* name, double* out, int64_t n) {

@@ -30,6 +30,29 @@ struct PrimPair {
   double p, oo2p, Px, Py, Pz, PAx, PAy, PAz, K, pad;
 };
 
+// Primitive pairs that every lane of a wavefront reads at the same address
+// (the ket list of a quartet kernel, the pair list of a Schwarz wave): through
+// the constant address space the compiler may load them with scalar loads into
+// SGPRs.  The data are written by an earlier kernel and read-only afterwards.
+typedef const __attribute__((address_space(4))) PrimPair* UniformPrimPairs;
+__device__ __forceinline__ UniformPrimPairs as_uniform(const PrimPair* p) { return (UniformPrimPairs)p; }
+// Field-wise copy out of the constant address space (the implicit copy
+// constructor binds only generic references); unused fields are dropped.
+__device__ __forceinline__ PrimPair load_uniform(UniformPrimPairs q) {
+  PrimPair r;
+  r.p = q->p;
+  r.oo2p = q->oo2p;
+  r.Px = q->Px;
+  r.Py = q->Py;
+  r.Pz = q->Pz;
+  r.PAx = q->PAx;
+  r.PAy = q->PAy;
+  r.PAz = q->PAz;
+  r.K = q->K;
+  r.pad = 0.0;
+  return r;
+}
+
 struct Shell {
   int atom, l, nprim, prim0, ao;
 };

@@ -29,7 +29,7 @@ int fail(int code, const std::string& msg);
   } while (0)
 
 struct PPSrc {
-  int pair, ia, ib, pad;  // pair index, global primitive indices of A and B
+  int sA, sB, ia, ib;  // shell indices of A and B, global primitive indices of A and B
 };
 
 struct OrbEntry {
@@ -102,6 +102,7 @@ struct vsvb_context {
   hipStream_t stream = nullptr;
   bool own_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (device entry)
   double tau = 1e-17;
   // sizes
   int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
@@ -110,6 +111,10 @@ struct vsvb_context {
   int pair_off[vsvb::kNPairClass + 1] = {0};
   // host copies
   std::vector<double> coords, charges, coeff;
+  // host copies for the per-geometry pair schedule
+  std::vector<Shell> h_shells;
+  std::vector<Pair> h_pairs;
+  std::vector<double> h_pexp, h_pcoef, sched_coords;
   std::vector<OrbEntry> ent;
   std::vector<int> orb_start;
   // device

@@ -164,6 +164,13 @@ class Context:
                "vsvb_energy_grad")
         return e.value, g
 
+    def energy_grad_device(self, coeff_ptr: int, energy_ptr: int, grad_ptr: int, stream: int = 0) -> None:
+        """E and dE/dcoeff with device pointers (e.g. torch CUDA tensors' data_ptr()),
+        ordered on the given HIP stream handle (vsvb_energy_grad_device)."""
+        _check(load().vsvb_energy_grad_device(self._h, C.c_void_p(coeff_ptr), C.c_void_p(energy_ptr),
+                                              C.c_void_p(grad_ptr), C.c_void_p(stream or None)),
+               "vsvb_energy_grad_device")
+
     def buffer(self, name: str, n: int) -> np.ndarray:
         """Copy of an internal device buffer (diagnostics)."""
         out = np.zeros(n)

@@ -325,16 +325,16 @@ def _type_shells(s: VsvbSystem, t: int):
     return shells
 
 
-def config4() -> VsvbSystem:
-    """LiF rock-salt slab 4x4x8 (128 atoms, 2.01 A); atom-centred docc:
-    Li 1s cores, F 1s cores and four F valence pairs."""
-    rng = np.random.default_rng(SEED_BASE + 4)
+def lif_slab(nx: int, ny: int, nz: int, seed: int = SEED_BASE + 4) -> VsvbSystem:
+    """LiF rock-salt slab nx x ny x nz (alternating Li/F, 2.01 A); atom-centred
+    docc: Li 1s cores, F 1s cores and four F valence pairs."""
+    rng = np.random.default_rng(seed)
     basis = {"Li": _basis_for(rng, "Li"), "F": _basis_for(rng, "F")}
     a = 2.01
     elems, xyz = [], []
-    for i in range(4):
-        for j in range(4):
-            for k in range(8):
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
                 elems.append("Li" if (i + j + k) % 2 == 0 else "F")
                 xyz.append([i * a, j * a, k * a])
     xyz = np.array(xyz)
@@ -343,7 +343,19 @@ def config4() -> VsvbSystem:
         nv = 1 if e == "Li" else 5
         for _ in range(nv):
             orbs.append(([at], _rand_orbital(rng, basis, elems, [at])))
-    return assemble(elems, xyz, basis, orbs, title="LiF 4x4x8 slab (seed 1004)")
+    return assemble(elems, xyz, basis, orbs, title=f"LiF {nx}x{ny}x{nz} slab (seed {seed})")
+
+
+def config4() -> VsvbSystem:
+    """LiF rock-salt slab 4x4x8 (128 atoms, 2.01 A)."""
+    return lif_slab(4, 4, 8)
+
+
+def lif8() -> VsvbSystem:
+    """The 2x2x2 corner of the LiF recipe (8 atoms, same seed and basis recipe):
+    small enough for the oracle's full energy."""
+    return lif_slab(2, 2, 2)
 
 
 CONFIGS = {0: config0, 1: config1, 2: config2, 3: config3, 4: config4}
+EXTRA = {"lif8": lif8}
