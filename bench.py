@@ -187,6 +187,7 @@ def main():
             "plain_energy_Eh": e_plain,
             "plain_eri_stage_ms": eri_ms_plain,
             "plain_eri_stage_tflops": flops_plain / (eri_ms_plain * 1e-3) / 1e12 if eri_ms_plain > 0 else None,
+            "batch_runs_replay_hip_graph": batch.uses_graph(),
         },
         "roofline": {
             "bound": "mfma",

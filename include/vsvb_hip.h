@@ -106,13 +106,21 @@ int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coef
 int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
                             double *grad_device, void *stream);
 
-/* Batched evaluation: many wave functions of uneven size in one graph launch. */
+/* Batched evaluation of many wave functions of uneven size (displaced
+ * geometries, trial orbital rotations, different molecules); no counterpart in
+ * the reference, which evaluates one wave function per process call.  Wave
+ * functions on one molecule share one pass over the shell quartets.  The first
+ * run is enqueued directly and then captured into a HIP graph; every later run
+ * is one graph launch (the systems are fixed at creation). */
 typedef struct vsvb_batch vsvb_batch;
 
 int vsvb_batch_create(int32_t nwfn, const vsvb_system *const *systems, const vsvb_options *opt, vsvb_batch **batch);
 /* energies: [nwfn]; grads: NULL or [nwfn] host pointers of totlen doubles each (NULL entries skipped). */
 int vsvb_batch_run(vsvb_batch *batch, int32_t want_grad, double *energies, double *const *grads);
 void vsvb_batch_destroy(vsvb_batch *batch);
+/* 1 when runs replay the captured graph, 0 when they are enqueued directly
+ * (before the first run, or after a failed capture: vsvb_last_error says why). */
+int vsvb_batch_uses_graph(vsvb_batch *batch);
 
 /* Diagnostics. */
 const char *vsvb_last_error(void);

@@ -150,3 +150,25 @@ def test_batch_groups_share_integrals(lib):
         print(f"wfn {i}: batch {e[k]:.12f} single {e1:.12f}")
         assert abs(e[k] - e1) < 1e-9
         assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)
+
+
+def test_batch_graph_replays(lib):
+    """The first batch run is direct and then captured into a HIP graph; replays,
+    with direct single evaluations between them, must reproduce it."""
+    systems = workloads.config3()
+    pick = [0, 48, 49, 90]
+    b = lib.Batch([systems[i] for i in pick])
+    e0, g0 = b.run(want_grad=True)
+    assert b.uses_graph(), b.graph_error()
+    for rep in range(3):
+        with lib.Context(systems[pick[rep]]) as ctx:
+            ctx.energy_grad()                      # direct work between replays
+        e, g = b.run(want_grad=True)
+        for k in range(len(pick)):
+            assert abs(e[k] - e0[k]) < 1e-10, (rep, k, e[k], e0[k])
+            assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])
+    for k, i in enumerate(pick):
+        with lib.Context(systems[i]) as ctx:
+            e1, g1 = ctx.energy_grad()
+        assert abs(e[k] - e1) < 1e-9
+        assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)

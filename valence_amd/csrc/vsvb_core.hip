@@ -534,6 +534,19 @@ hipError_t vsvb::launch_fill(hipStream_t st, double* p, size_t n, double v) {
   return hipGetLastError();
 }
 
+// Clearing counters, flags and status words is done by a kernel, not a memset,
+// so that the same code can be captured into a HIP graph.
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t vsvb::launch_zero(hipStream_t st, void* p, size_t bytes) {
+  const size_t n = bytes / sizeof(uint32_t);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>(256, (n + 255) / 256 + 1)), dim3(256), 0, st,
+                     (uint32_t*)p, n, 0u);
+  return hipGetLastError();
+}
+
 hipError_t vsvb::launch_reduce(hipStream_t st, const double* partial, long n, int nw, double* out) {
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, n, nw, out);
   return hipGetLastError();
@@ -914,7 +927,8 @@ int vsvb::prepare_geometry(vsvb_context* c, const double* coords) {
   int rc;
   if ((rc = schedule_pairs(c))) return rc;
   c->enuc = nuclear_repulsion(c->natom, c->coords.data(), c->charges.data());
-  HIP_OK(hipMemcpyAsync(c->d_xyz.p, c->coords.data(), 3 * c->natom * sizeof(double), hipMemcpyHostToDevice, st));
+  if (!c->capturing)
+    HIP_OK(hipMemcpyAsync(c->d_xyz.p, c->coords.data(), 3 * c->natom * sizeof(double), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_pair_geom, dim3((c->npairs + 255) / 256), dim3(256), 0, st, c->d_pairs.p, c->npairs,
                      c->d_shells.p, c->d_xyz.p);
   HIP_OK(hipGetLastError());
@@ -948,7 +962,7 @@ int vsvb::prepare_orbital_basis(vsvb_context* c, const double* coeff_host, const
   }
   const int n = c->nbf, ns = c->nsp;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->d_scal.p, (size_t)16, 0.0);
-  HIP_OK(hipMemsetAsync(c->d_status.p, 0, 4 * sizeof(int), st));
+  HIP_OK(launch_zero(st, c->d_status.p, 4 * sizeof(int)));
   if (c->ndf > 0) {
     hipLaunchKernelGGL(k_ndf_scale, dim3((c->ndf + 63) / 64), dim3(64), 0, st, c->d_ent.p, c->d_orb_start.p,
                        c->d_coeff.p, c->d_angn.p, c->d_S.p, n, c->first_ndf, c->ndf, c->d_ndf_scale.p);
@@ -1012,8 +1026,8 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
   if (!dens) dens = c->d_R.p;
   if (!jacc) jacc = c->d_J.p;
   if (!kacc) kacc = c->d_K.p;
-  HIP_OK(hipMemsetAsync(c->d_counters.p, 0, 4 * kNQuartetClass * sizeof(unsigned long long), st));
-  HIP_OK(hipMemsetAsync(c->d_tilectr.p, 0, kNQuartetClass * sizeof(int), st));
+  HIP_OK(launch_zero(st, c->d_counters.p, 4 * kNQuartetClass * sizeof(unsigned long long)));
+  HIP_OK(launch_zero(st, c->d_tilectr.p, kNQuartetClass * sizeof(int)));
   c->last_jk = jk;
   c->last_nwfn = nwfn;
   c->last_nspin = nspin;
@@ -1075,7 +1089,7 @@ static double digestion_flops(int q, int nspin, bool jk) {
   return jk ? nab * ncd * (4.0 + 8.0 * nspin) : nab * ncd * (2.0 + 4.0 * nspin) + 2.0 * nab;
 }
 
-int vsvb::finish_counters(vsvb_context* c) {
+int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
   if (!c->ran_quartets) {  // a batch-group member: its integrals were counted by the leader
     std::memset(c->class_stats, 0, sizeof(c->class_stats));
     std::memset(c->last_counters, 0, sizeof(c->last_counters));
@@ -1083,7 +1097,10 @@ int vsvb::finish_counters(vsvb_context* c) {
   }
   c->ran_quartets = false;
   unsigned long long h[4 * kNQuartetClass];
-  HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
+  if (hcnt)
+    std::memcpy(h, hcnt, sizeof(h));
+  else
+    HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   const int nspin = c->last_nspin;
@@ -1240,16 +1257,151 @@ struct BatchGroup {
   DevBuf<double> dens, jacc, kacc;
 };
 
+// The first run of a batch is enqueued eagerly; the second captures the whole
+// run (every group's geometry, orbital, quartet and gradient launches, then a
+// gather of all energies, gradients, status words and counters into four
+// device arrays) into one HIP graph, and later runs replay it: one graph launch
+// and one copy to pinned host memory per run.  The systems of a batch are fixed
+// at creation, so the captured work never changes.
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
   std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;
+  int runs = 0;
+  bool graph_failed = false;
+  std::string graph_error;
+  hipGraphExec_t exec = nullptr;
+  std::vector<size_t> goff;      // offset of each wave function's gradient in d_g
+  std::vector<char> ran;         // the wave function ran the quartet stage itself (group leader)
+  DevBuf<double> d_e, d_g;
+  DevBuf<int> d_st;
+  DevBuf<unsigned long long> d_cnt;
+  double* h_e = nullptr;
+  double* h_g = nullptr;
+  int* h_st = nullptr;
+  unsigned long long* h_cnt = nullptr;
   ~vsvb_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (h_e) (void)hipHostFree(h_e);
+    if (h_g) (void)hipHostFree(h_g);
+    if (h_st) (void)hipHostFree(h_st);
+    if (h_cnt) (void)hipHostFree(h_cnt);
     groups.clear();
     for (auto* c : ctx) vsvb_context_destroy(c);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+constexpr int kCnt = 4 * kNQuartetClass;  // counters per context
+
+// All device work of one batch run, no host synchronization.
+static int batch_enqueue(vsvb_batch* b) {
+  int rc;
+  // Every wave function goes through the J/K (adjoint-ready) contraction; the
+  // energy is assembled from it, and the gradient when asked for.
+  for (auto& g : b->groups) {
+    vsvb_context* lead = b->ctx[g.members[0]];
+    for (int i : g.members)
+      for (int k = 0; k < b->ctx[i]->totlen; ++k)
+        if (b->ctx[i]->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
+    // geometry-dependent data are rebuilt every run: passing the coordinates forces it
+    if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
+    if (lead->npair > 0) {
+      if ((rc = prepare_orbital_basis(lead, nullptr, nullptr))) return rc;
+      if ((rc = sc_enqueue(lead, true, lead->d_grad.p))) return rc;
+      continue;
+    }
+    for (int i : g.members)
+      if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
+    if (g.members.size() > 1) {
+      if ((rc = run_quartets(lead, true, nullptr, (int)g.members.size(), g.dens.p, g.jacc.p, g.kacc.p))) return rc;
+    } else if ((rc = run_quartets(lead, true, nullptr))) {
+      return rc;
+    }
+    for (int i : g.members) {
+      vsvb_context* m = b->ctx[i];
+      m->enuc = lead->enuc;
+      if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
+    }
+  }
+  return VSVB_OK;
+}
+
+// Results of every wave function into the batch's gather arrays.
+static int batch_gather(vsvb_batch* b) {
+  hipStream_t st = b->stream;
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    vsvb_context* c = b->ctx[i];
+    if (c->npair > 0) {
+      HIP_OK(hipMemcpyAsync(b->d_e.p + i, c->d_scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      hipLaunchKernelGGL(k_final_energy, dim3(1), dim3(1), 0, st, c->d_scal.p, c->d_scal.p + 3, c->enuc, b->d_e.p + i);
+      HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipMemcpyAsync(b->d_st.p + 4 * i, c->d_status.p, 4 * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(b->d_g.p + b->goff[i], c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(b->d_cnt.p + kCnt * i, c->d_counters.p, kCnt * sizeof(unsigned long long),
+                          hipMemcpyDeviceToDevice, st));
+  }
+  return VSVB_OK;
+}
+
+static int batch_capture(vsvb_batch* b) {
+  for (auto* c : b->ctx) c->capturing = true;
+  int rc = VSVB_OK;
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = batch_enqueue(b);
+    if (!rc) rc = batch_gather(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (!rc && e2 == hipSuccess && g) {
+      e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) b->exec = nullptr;
+    } else if (e2 != hipSuccess) {
+      e = e2;
+    }
+    if (g) (void)hipGraphDestroy(g);
+  }
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    b->ran[i] = b->ctx[i]->ran_quartets;
+    b->ctx[i]->capturing = false;
+  }
+  if (rc || e != hipSuccess || !b->exec) {
+    b->graph_failed = true;
+    b->graph_error = rc ? g_last_error : std::string(hipGetErrorString(e));
+    (void)hipGetLastError();
+    return VSVB_ERR_DEVICE;
+  }
+  return VSVB_OK;
+}
+
+static int batch_replay(vsvb_batch* b, int32_t want_grad, double* energies, double* const* grads) {
+  hipStream_t st = b->stream;
+  const size_t n = b->ctx.size();
+  HIP_OK(hipGraphLaunch(b->exec, st));
+  HIP_OK(hipMemcpyAsync(b->h_e, b->d_e.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(b->h_st, b->d_st.p, 4 * n * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(b->h_g, b->d_g.p, b->goff[n] * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(b->h_cnt, b->d_cnt.p, kCnt * n * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < n; ++i) {
+    vsvb_context* c = b->ctx[i];
+    const int* s4 = b->h_st + 4 * i;
+    if (s4[0]) return fail(VSVB_ERR_NUMERIC, "wave function " + std::to_string(i) + ": orbital overlap block is not positive definite");
+    if (s4[2]) return fail(VSVB_ERR_NUMERIC, "wave function " + std::to_string(i) + ": exactly singular spin-coupled transition overlap");
+    if (want_grad && s4[1]) return fail(VSVB_ERR_UNSUPPORTED, "wave function " + std::to_string(i) + ": gradient with masked shells");
+    if (want_grad && c->npair > 0 && s4[3])
+      return fail(VSVB_ERR_UNSUPPORTED, "wave function " + std::to_string(i) + ": gradient at a singular spin-coupled transition overlap");
+    if (!std::isfinite(b->h_e[i])) return fail(VSVB_ERR_NUMERIC, "wave function " + std::to_string(i) + ": non-finite energy");
+    if (energies) energies[i] = b->h_e[i];
+    if (want_grad && grads && grads[i]) std::memcpy(grads[i], b->h_g + b->goff[i], c->totlen * sizeof(double));
+    c->ran_quartets = b->ran[i];
+    int rc;
+    if ((rc = finish_counters(c, b->h_cnt + kCnt * i))) return rc;
+  }
+  return VSVB_OK;
+}
 
 static bool same_molecule(const vsvb_system* a, const vsvb_system* b) {
   if (a->natom != b->natom || a->ntype != b->ntype || a->nshell_type != b->nshell_type || a->nprim != b->nprim)
@@ -1522,6 +1674,18 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
       m->d_K.alias(g.kacc.p + k * nspin * nn, nspin * nn);
     }
   }
+  // gather arrays for graph replays
+  bt->goff.assign(nwfn + 1, 0);
+  for (int i = 0; i < nwfn; ++i) bt->goff[i + 1] = bt->goff[i] + std::max(1, bt->ctx[i]->totlen);
+  bt->ran.assign(nwfn, 0);
+  HIP_OK(bt->d_e.alloc(nwfn));
+  HIP_OK(bt->d_g.alloc(bt->goff[nwfn]));
+  HIP_OK(bt->d_st.alloc(4 * (size_t)nwfn));
+  HIP_OK(bt->d_cnt.alloc((size_t)kCnt * nwfn));
+  HIP_OK(hipHostMalloc((void**)&bt->h_e, nwfn * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&bt->h_g, bt->goff[nwfn] * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&bt->h_st, 4 * (size_t)nwfn * sizeof(int), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&bt->h_cnt, (size_t)kCnt * nwfn * sizeof(unsigned long long), hipHostMallocDefault));
   *b = bt;
   return VSVB_OK;
 }
@@ -1529,41 +1693,26 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
 int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* const* grads) {
   if (!b) return fail(VSVB_ERR_ARG, "null batch");
   int rc;
-  // Every wave function goes through the J/K (adjoint-ready) contraction; the
-  // energy is assembled from it, and the gradient when asked for.
-  for (auto& g : b->groups) {
-    vsvb_context* lead = b->ctx[g.members[0]];
-    HIP_OK(hipSetDevice(lead->device));
-    for (int i : g.members)
-      for (int k = 0; k < lead->totlen && i == g.members[0]; ++k)
-        if (lead->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
-    // geometry-dependent data are rebuilt every run: passing the coordinates forces it
-    if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
-    if (lead->npair > 0) {
-      if ((rc = prepare_orbital_basis(lead, nullptr, nullptr))) return rc;
-      if ((rc = sc_enqueue(lead, true, lead->d_grad.p))) return rc;
-      continue;
-    }
-    for (int i : g.members)
-      if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
-    if (g.members.size() > 1) {
-      if ((rc = run_quartets(lead, true, nullptr, (int)g.members.size(), g.dens.p, g.jacc.p, g.kacc.p))) return rc;
-    } else if ((rc = run_quartets(lead, true, nullptr))) {
-      return rc;
-    }
-    for (int i : g.members) {
-      vsvb_context* m = b->ctx[i];
-      m->enuc = lead->enuc;
-      if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
-    }
-  }
+  HIP_OK(hipSetDevice(b->ctx[0]->device));
+  ++b->runs;
+  if (b->exec) return batch_replay(b, want_grad, energies, grads);
+  if ((rc = batch_enqueue(b))) return rc;
   for (size_t i = 0; i < b->ctx.size(); ++i) {
     vsvb_context* c = b->ctx[i];
     double* e = energies ? energies + i : nullptr;
     double* g = (want_grad && grads) ? grads[i] : nullptr;
     if ((rc = c->npair > 0 ? sc_finish(c, e, nullptr, g) : finish_energy_grad(c, e, g))) return rc;
   }
+  // capturing runs nothing: the graph is built now, after the first (direct) run,
+  // and replayed from the next run on; on failure runs stay direct
+  if (!b->graph_failed) (void)batch_capture(b);
   return VSVB_OK;
+}
+
+int vsvb_batch_uses_graph(vsvb_batch* b) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  if (b->graph_failed) g_last_error = "graph capture failed: " + b->graph_error;
+  return b->exec ? 1 : 0;
 }
 
 void vsvb_batch_destroy(vsvb_batch* b) { delete b; }

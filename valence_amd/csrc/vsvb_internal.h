@@ -144,6 +144,7 @@ struct vsvb_context {
   bool last_jk = false;
   int last_nwfn = 1, last_nspin = 1;  // densities and spin blocks of the last quartet launch
   bool ran_quartets = false;  // this context launched the quartet stage in the current call
+  bool capturing = false;     // enqueueing into a HIP graph capture: no host uploads
   hipEvent_t cls_ev[vsvb::kNQuartetClass + 1] = {};
   double class_stats[vsvb::kNQuartetClass][6] = {};
   // spin-coupled wave functions (npair > 0): determinant lists and work arrays
@@ -156,13 +157,14 @@ namespace vsvb {
 hipError_t gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
                 const double* B, int ldb, bool tb, double beta, double* C, int ldc);
 hipError_t launch_fill(hipStream_t st, double* p, size_t n, double v);
+hipError_t launch_zero(hipStream_t st, void* p, size_t bytes);  // bytes: a multiple of 4
 hipError_t launch_reduce(hipStream_t st, const double* partial, long n, int nw, double* out);
 hipError_t launch_gather_grad(vsvb_context* c, const double* gradC, double* grad);
 int prepare_geometry(vsvb_context* c, const double* coords);
 int prepare_orbital_basis(vsvb_context* c, const double* coeff_host, const double* coeff_dev);
 int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, const double* dens = nullptr,
                  double* jacc = nullptr, double* kacc = nullptr, int nspin_override = -1);
-int finish_counters(vsvb_context* c);
+int finish_counters(vsvb_context* c, const unsigned long long* host_counters = nullptr);
 int check_status(vsvb_context* c, bool for_grad);
 
 // spin-coupled path (vsvb_sc.hip)

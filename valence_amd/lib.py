@@ -47,7 +47,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
-           "vsvb_version"]
+           "vsvb_version", "vsvb_batch_uses_graph"]
 
 _lib = None
 
@@ -81,6 +81,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_batch_class_stats.argtypes = [C.c_void_p, _f64p]
     L.vsvb_version.restype = C.c_char_p
+    L.vsvb_batch_uses_graph.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -216,6 +217,17 @@ class Batch:
 
     def set_profiling(self, on: bool = True):
         _check(load().vsvb_batch_set_profiling(self._h, int(on)), "vsvb_batch_set_profiling")
+
+    def uses_graph(self) -> bool:
+        """True once runs replay the captured HIP graph (captured after the first run)."""
+        r = load().vsvb_batch_uses_graph(self._h)
+        if r < 0:
+            _check(r, "vsvb_batch_uses_graph")
+        return bool(r)
+
+    def graph_error(self) -> str:
+        load().vsvb_batch_uses_graph(self._h)
+        return load().vsvb_last_error().decode(errors="replace")
 
     def class_stats(self) -> np.ndarray:
         out = np.zeros((21, 6))
