@@ -19,8 +19,8 @@ LIB = os.path.join(HERE, "libvsvb_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["vsvb_core.hip", "vsvb_sc.hip"] + [f"vsvb_eri_b{b}.hip" for b in range(6)]
-ERI_HEADERS = ["vsvb_device.h", "vsvb_eri_kernels.h", "eri_gen.inc"]
+SOURCES = ["vsvb_core.hip", "vsvb_sc.hip", "vsvb_eri_coop.hip"] + [f"vsvb_eri_b{b}.hip" for b in range(6)]
+ERI_HEADERS = ["vsvb_device.h", "vsvb_eri_kernels.h", "eri_gen.inc", "coop_tables.inc"]
 HOST_HEADERS = ["vsvb_internal.h", "eri_flops.inc", os.path.join("..", "..", "include", "vsvb_hip.h")]
 
 # Packed FP32 instructions are disabled for all device code (they have given

@@ -83,9 +83,9 @@ class Emitter:
         self.lines.append(s)
 
 
-def gen_vrr(la, lb, lc, ld, em: Emitter):
-    """Emit VRR for one primitive quartet; returns dict target->(var name)."""
-    L = la + lb + lc + ld
+def vrr_plan(la, lb, lc, ld):
+    """Intermediates [e0|f0]^(m) needed for the class, in dependency order, with
+    the recurrence used for each (greedy choice of the decrement direction)."""
     targets = [(e, f) for le in range(la, la + lb + 1) for e in cart(le)
                for lf in range(lc, lc + ld + 1) for f in cart(lf)]
     order: List[Tuple] = []
@@ -147,6 +147,12 @@ def gen_vrr(la, lb, lc, ld, em: Emitter):
 
     for (e, f) in targets:
         need(e, f, 0)
+    return order, recipe, targets
+
+
+def gen_vrr(la, lb, lc, ld, em: Emitter):
+    """Emit VRR for one primitive quartet; returns dict target->(var name)."""
+    order, recipe, targets = vrr_plan(la, lb, lc, ld)
 
     def var(k):
         e, f, m = k
@@ -324,6 +330,221 @@ def gen_class(la, lb, lc, ld):
     return "\n".join(em.lines), info
 
 
+# Classes evaluated by the cooperative kernel (one wavefront per contracted
+# quartet, intermediates in LDS): the (dd| classes with p or d kets, whose
+# one-lane-per-quartet code spills thousands of registers and whose quartets are
+# few.  Classes with many smaller quartets ((dp|pp), (dp|dp), (dd|ds)) measured
+# faster on the straight-line kernels despite their spills.
+COOP_CLASSES = [(2, 2, 1, 1), (2, 2, 2, 1), (2, 2, 2, 2)]
+
+
+def coop_plan(la, lb, lc, ld):
+    """Step tables of one class for the cooperative kernel.
+
+    LDS slots: [0, nT) the contracted targets [e0|f0] (accumulated over primitive
+    quartets); [nT, nT + nV) the VRR intermediates of one primitive quartet;
+    after the primitive loop the HRR intermediates reuse [nT, nT + nH).
+    VRR step (8 ints): out, a, b, c, d, e, code, 0 with
+      code = kind | dir << 2 | n1 << 4 | n2 << 8, kind 1 = bra, 2 = ket:
+      bra: out = PA_dir S[a] + WP_dir S[b] + n1 oo2p (S[c] - rop S[d])
+      ket: out = QC_dir S[a] + WQ_dir S[b] + n1 oo2q (S[c] - roq S[d]) + n2 oo2s S[e]
+    (absent terms: n1 or n2 = 0).  Base (2 ints): slot, m: S[slot] = pref F_m.
+    HRR step (4 ints): out, a, b, code = dir | 4 * (ket): out = S[a] + (AB or CD)_dir S[b].
+    Steps are grouped in levels; a level only reads earlier levels."""
+    order, recipe, targets = vrr_plan(la, lb, lc, ld)
+    nT = len(targets)
+    acc = {t: i for i, t in enumerate(targets)}
+    vslot = {k: nT + i for i, k in enumerate(order)}
+    base, lev, vsteps = [], {}, []
+    for k in order:
+        r = recipe[k]
+        if r[0] == "base":
+            base.append((vslot[k], r[1]))
+            lev[k] = 0
+            continue
+        a = b = c = d = e = -1
+        n1 = n2 = 0
+        deps = []
+        for part in r[1]:
+            if part[0] in ("PA", "QC"):
+                a = vslot[part[2]]; deps.append(part[2]); dr = part[1]
+            elif part[0] in ("WP", "WQ"):
+                b = vslot[part[2]]; deps.append(part[2])
+            elif part[0] in ("E2P", "F2Q"):
+                n1 = part[2]; c = vslot[part[3]]; d = vslot[part[4]]; deps += [part[3], part[4]]
+            elif part[0] == "E2S":
+                n2 = part[2]; e = vslot[part[3]]; deps.append(part[3])
+        kind = 1 if r[0] == "bra" else 2
+        lev[k] = 1 + max(lev[x] for x in deps)
+        vsteps.append((lev[k], (vslot[k], a, b, c, d, e, kind | dr << 2 | n1 << 4 | n2 << 8, 0)))
+    vsteps.sort(key=lambda x: x[0])
+    nvl = max([l for l, _ in vsteps] + [0])
+    vlev = [0] * (nvl + 1)
+    for l, _ in vsteps:
+        vlev[l] += 1
+    voff = [0]
+    for l in range(1, nvl + 1):
+        voff.append(voff[-1] + vlev[l])
+    targ = [(acc[t], vslot[(t[0], t[1], 0)]) for t in targets]
+    # HRR
+    fs = [f for lf in range(lc, lc + ld + 1) for f in cart(lf)]
+    nxt = [nT]
+    hlev_of = {}
+    hsteps = []
+
+    def new_slot():
+        v = nxt[0]
+        nxt[0] += 1
+        return v
+
+    bseen, border = {}, []
+
+    def bra(a_, b_):
+        key = (a_, b_)
+        if key in bseen:
+            return key
+        if sum(b_) == 0:
+            bseen[key] = ("base",)
+        else:
+            i = next(i for i in range(3) if b_[i] > 0)
+            b1 = sub(b_, i)
+            bseen[key] = ("rec", i, bra(add(a_, i), b1), bra(a_, b1))
+        border.append(key)
+        return key
+
+    for a_ in cart(la):
+        for b_ in cart(lb):
+            bra(a_, b_)
+    bslot = {}
+
+    def bvar(key, f):
+        if sum(key[1]) == 0:
+            return acc[(key[0], f)]
+        return bslot[(key, f)]
+
+    for key in border:
+        r = bseen[key]
+        if r[0] == "base":
+            continue
+        for f in fs:
+            o = new_slot()
+            bslot[(key, f)] = o
+            sa, sb = bvar(r[2], f), bvar(r[3], f)
+            hlev_of[o] = 1 + max(hlev_of.get(sa, 0), hlev_of.get(sb, 0))
+            hsteps.append((hlev_of[o], (o, sa, sb, r[1])))
+    kseen, korder = {}, []
+
+    def ket(c_, d_):
+        key = (c_, d_)
+        if key in kseen:
+            return key
+        if sum(d_) == 0:
+            kseen[key] = ("base",)
+        else:
+            i = next(i for i in range(3) if d_[i] > 0)
+            d1 = sub(d_, i)
+            kseen[key] = ("rec", i, ket(add(c_, i), d1), ket(c_, d1))
+        korder.append(key)
+        return key
+
+    for c_ in cart(lc):
+        for d_ in cart(ld):
+            ket(c_, d_)
+    kslot = {}
+
+    def kvar(ab, key):
+        if sum(key[1]) == 0:
+            return bvar(ab, key[0])
+        return kslot[(ab, key)]
+
+    out_slot = []
+    for a_ in cart(la):
+        for b_ in cart(lb):
+            ab = (a_, b_)
+            for key in korder:
+                r = kseen[key]
+                if r[0] == "base":
+                    continue
+                o = new_slot()
+                kslot[(ab, key)] = o
+                sa, sb = kvar(ab, r[2]), kvar(ab, r[3])
+                hlev_of[o] = 1 + max(hlev_of.get(sa, 0), hlev_of.get(sb, 0))
+                hsteps.append((hlev_of[o], (o, sa, sb, r[1] | 4)))
+            for c_ in cart(lc):
+                for d_ in cart(ld):
+                    out_slot.append(kvar(ab, (c_, d_)))
+    hsteps.sort(key=lambda x: x[0])
+    nhl = max([l for l, _ in hsteps] + [0])
+    hcount = [0] * (nhl + 1)
+    for l, _ in hsteps:
+        hcount[l] += 1
+    hoff = [0]
+    for l in range(1, nhl + 1):
+        hoff.append(hoff[-1] + hcount[l])
+    nV, nH = len(order), nxt[0] - nT
+    return dict(nT=nT, nV=nV, nH=nH, lds=nT + max(nV, nH), base=base, voff=voff,
+                vsteps=[st for _, st in vsteps], targ=targ, hoff=hoff, hsteps=[st for _, st in hsteps],
+                out=out_slot)
+
+
+def write_coop(path: str) -> None:
+    """coop_tables.inc: one flat int array and a descriptor per cooperative class."""
+    data: List[int] = []
+    descs = []
+
+    def align4():  # every section starts on a 16-byte boundary (the kernel reads int4 records)
+        while len(data) % 4:
+            data.append(0)
+
+    for cls in COOP_CLASSES:
+        t = coop_plan(*cls)
+        d = dict(cls=cls, nT=t["nT"], nV=t["nV"], nH=t["nH"], lds=t["lds"])
+        align4()
+        d["base"] = len(data); d["nbase"] = len(t["base"])
+        for sl, m in t["base"]:
+            data += [sl, m]
+        align4()
+        d["voff"] = len(data); d["nvl"] = len(t["voff"]) - 1
+        data += t["voff"]
+        align4()
+        d["vsteps"] = len(data)
+        for st in t["vsteps"]:
+            data += list(st)
+        align4()
+        d["targ"] = len(data)
+        for a_, v_ in t["targ"]:
+            data += [a_, v_]
+        align4()
+        d["hoff"] = len(data); d["nhl"] = len(t["hoff"]) - 1
+        data += t["hoff"]
+        align4()
+        d["hsteps"] = len(data)
+        for st in t["hsteps"]:
+            data += list(st)
+        align4()
+        d["out"] = len(data)
+        data += t["out"]
+        descs.append(d)
+    lines = ["// Generated by gen_eri.py -- do not edit.",
+             "// Step tables of the cooperative (one wavefront per quartet) ERI kernel.",
+             f"constexpr int kNCoop = {len(descs)};",
+             f"constexpr int kCoopDataLen = {len(data)};",
+             "alignas(16) static const int kCoopData[kCoopDataLen] = {"]
+    for i in range(0, len(data), 24):
+        lines.append("  " + ", ".join(str(x) for x in data[i:i + 24]) + ",")
+    lines.append("};")
+    lines.append("static const CoopDesc kCoopDesc[kNCoop] = {")
+    for d in descs:
+        la, lb, lc, ld = d["cls"]
+        lines.append(f"  {{{la}, {lb}, {lc}, {ld}, {d['nT']}, {d['nV']}, {d['nH']}, {d['lds']}, {d['nbase']}, {d['base']}, "
+                     f"{d['nvl']}, {d['voff']}, {d['vsteps']}, {d['targ']}, {d['nhl']}, {d['hoff']}, {d['hsteps']}, {d['out']}}},")
+    lines.append("};")
+    text = "\n".join(lines) + "\n"
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+
+
 def main(out_path: str) -> Dict:
     chunks = ["// Generated by gen_eri.py -- do not edit.", "",
               "template <int LA, int LB, int LC, int LD>",
@@ -364,4 +585,5 @@ def main(out_path: str) -> Dict:
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))
     infos = main(os.path.join(here, "eri_gen.inc"))
+    write_coop(os.path.join(here, "coop_tables.inc"))
     json.dump(infos, sys.stdout, indent=1)

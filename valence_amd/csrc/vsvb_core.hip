@@ -834,11 +834,27 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   HIP_OK(c->d_tilectr.alloc(kNQuartetClass));
+  HIP_OK(coop_prepare(c->device));
   c->partial_total = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
       const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+      const int ci = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
+      c->coop_idx[q] = ci;
+      if (ci >= 0) {  // one wavefront (workgroup) per contracted quartet
+        const int units = (nb > 0 && nk > 0) ? nb * nk : 0;
+        c->ntiles_class[q] = units;
+        c->ntiles_jk[q] = units;
+        // an empty class launches nothing and owns no energy partials
+        c->blocks_per_class[q] = units > 0 ? std::min(units, coop_occupancy(ci, false) * ncu) : 0;
+        c->blocks_jk[q] = units > 0 ? std::min(units, coop_occupancy(ci, true) * ncu) : 0;
+        c->coop_claim[q] = units > 0 ? std::max(1, std::min(64, units / (4 * c->blocks_jk[q]))) : 1;
+        c->lds_jk[q] = 0;
+        c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;  // the kernel writes four slots per block
+        c->partial_total += c->partial_per_class[q];
+        continue;
+      }
       const int tb = (nb + kTileB - 1) / kTileB;
       c->ntiles_class[q] = (nb > 0 && nk > 0) ? tb * nk : 0;  // energy work units, one wavefront each
       const int waves = (c->ntiles_class[q] + 3) / 4;
@@ -1051,7 +1067,7 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       a.ket0 = c->pair_off[kc];
       a.nket = c->pair_off[kc + 1] - a.ket0;
       a.same = bc == kc;
-      a.ntk = (a.nket + kTileK - 1) / kTileK;
+      a.ntk = c->coop_claim[q];
       a.ntiles = jk ? c->ntiles_jk[q] : c->ntiles_class[q];
       a.lds_k = jk && c->lds_jk[q] > 0;
       a.tau = c->tau;
@@ -1065,8 +1081,11 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       a.counters = c->d_counters.p + 4 * q;
       a.tile_ctr = c->d_tilectr.p + q;
       if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
-      HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q],
-                                  jk ? c->lds_jk[q] : 0, st));
+      if (c->coop_idx[q] >= 0)
+        HIP_OK(launch_coop(c->coop_idx[q], a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q], c->device, st));
+      else
+        HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q],
+                                    jk ? c->lds_jk[q] : 0, st));
       poff += c->partial_per_class[q];
     }
   if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));

@@ -37,7 +37,7 @@ struct QuartetArgs {
   int bra0, nbra;       // pair index range of the bra class
   int ket0, nket;       // pair index range of the ket class
   int same;             // bra class == ket class: only ket <= bra
-  int ntk;              // unused
+  int ntk;              // cooperative kernel: units claimed per wavefront (1..64)
   int ntiles;           // work units: energy: bra tiles (64 pairs) x ket pairs; J/K: bra strips (256) x ket pairs
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;

@@ -134,6 +134,8 @@ struct vsvb_context {
   int blocks_per_class[vsvb::kNQuartetClass] = {0};
   int ntiles_class[vsvb::kNQuartetClass] = {0};
   int ntiles_jk[vsvb::kNQuartetClass] = {0};      // J/K work units (256-pair bra strips x ket pairs)
+  int coop_idx[vsvb::kNQuartetClass] = {0};       // >= 0: class runs on the cooperative kernel
+  int coop_claim[vsvb::kNQuartetClass] = {0};     // cooperative kernel: units claimed per wavefront
   size_t lds_jk[vsvb::kNQuartetClass] = {0};     // J/K exchange LDS block per workgroup (0: global atomics)
   size_t partial_total = 0;
   bool geom_ready = false;
@@ -166,6 +168,12 @@ int run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn = 1, const d
                  double* jacc = nullptr, double* kacc = nullptr, int nspin_override = -1);
 int finish_counters(vsvb_context* c, const unsigned long long* host_counters = nullptr);
 int check_status(vsvb_context* c, bool for_grad);
+
+// cooperative kernel for the register-heavy classes (vsvb_eri_coop.hip)
+int coop_class(int la, int lb, int lc, int ld);  // index of the class, or -1
+int coop_occupancy(int idx, bool jk);
+hipError_t coop_prepare(int device);              // step tables resident on the device
+hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int device, hipStream_t st);
 
 // spin-coupled path (vsvb_sc.hip)
 int sc_setup(vsvb_context* c, const vsvb_system* s);
