@@ -70,6 +70,20 @@ def cpu_baseline(budget_s: float):
             "energy_seconds_extrapolated": t_energy}
 
 
+def tuner_summary(t) -> dict:
+    """Class -> chosen J/K launch ('coop' or 'lds'/'atomic' x workgroups per CU) and its time vs the default."""
+    out = {}
+    for q in range(21):
+        if t[q, 4] <= 0:
+            continue
+        if t[q, 0]:
+            kind = f"coop claim={int(t[q, 3]) or 'auto'}"
+        else:
+            kind = f"{'lds' if t[q, 2] else 'atomic'} wg/cu={int(t[q, 1]) or 'occ'}"
+        out[CLASS_NAMES[q]] = f"{kind} {t[q, 4]:.3f} ms (default {t[q, 5]:.3f})"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +202,7 @@ def main():
             "plain_eri_stage_ms": eri_ms_plain,
             "plain_eri_stage_tflops": flops_plain / (eri_ms_plain * 1e-3) / 1e12 if eri_ms_plain > 0 else None,
             "batch_runs_replay_hip_graph": batch.uses_graph(),
+            "batch_tuner": tuner_summary(batch.tuning()),
         },
         "roofline": {
             "bound": "mfma",

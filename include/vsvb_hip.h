@@ -121,6 +121,11 @@ void vsvb_batch_destroy(vsvb_batch *batch);
 /* 1 when runs replay the captured graph, 0 when they are enqueued directly
  * (before the first run, or after a failed capture: vsvb_last_error says why). */
 int vsvb_batch_uses_graph(vsvb_batch *batch);
+/* Launch choices of the batch tuner, which runs before the first run: for every
+ * class (J/K mode), out[21][6] = {cooperative kernel 0/1, workgroups per CU
+ * (0 = occupancy), LDS exchange staging 0/1, units per claim (0 = automatic),
+ * best ms, default-configuration ms}; zeros for classes without work. */
+int vsvb_batch_tuning(vsvb_batch *batch, double *out);
 
 /* Diagnostics. */
 const char *vsvb_last_error(void);

@@ -335,7 +335,11 @@ def gen_class(la, lb, lc, ld):
 # one-lane-per-quartet code spills thousands of registers and whose quartets are
 # few.  Classes with many smaller quartets ((dp|pp), (dp|dp), (dd|ds)) measured
 # faster on the straight-line kernels despite their spills.
-COOP_CLASSES = [(2, 2, 1, 1), (2, 2, 2, 1), (2, 2, 2, 2)]
+COOP_DEFAULT = [(2, 2, 1, 1), (2, 2, 2, 1), (2, 2, 2, 2)]
+# Tables exist for every class whose straight-line code spills; the batch
+# tuner may move the others onto the cooperative kernel.
+COOP_CLASSES = [(2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 1, 0), (2, 2, 1, 1), (2, 2, 2, 0),
+                (2, 2, 2, 1), (2, 2, 2, 2)]
 
 
 def coop_plan(la, lb, lc, ld):
@@ -536,8 +540,10 @@ def write_coop(path: str) -> None:
     lines.append("static const CoopDesc kCoopDesc[kNCoop] = {")
     for d in descs:
         la, lb, lc, ld = d["cls"]
+        dflt = 1 if d["cls"] in COOP_DEFAULT else 0
         lines.append(f"  {{{la}, {lb}, {lc}, {ld}, {d['nT']}, {d['nV']}, {d['nH']}, {d['lds']}, {d['nbase']}, {d['base']}, "
-                     f"{d['nvl']}, {d['voff']}, {d['vsteps']}, {d['targ']}, {d['nhl']}, {d['hoff']}, {d['hsteps']}, {d['out']}}},")
+                     f"{d['nvl']}, {d['voff']}, {d['vsteps']}, {d['targ']}, {d['nhl']}, {d['hoff']}, {d['hsteps']}, {d['out']}, "
+                     f"{dflt}}},")
     lines.append("};")
     text = "\n".join(lines) + "\n"
     if not os.path.exists(path) or open(path).read() != text:

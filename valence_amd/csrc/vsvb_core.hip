@@ -650,6 +650,53 @@ static hipError_t launch_schwarz(int pc, Pair* pairs, const PrimPair* pp, const 
 
 static int quartet_class_index(int bc, int kc) { return bc * (bc + 1) / 2 + kc; }
 
+static int class_bra(int q) {
+  for (int b = 0; b < kNPairClass; ++b)
+    if (q < quartet_class_index(b, b) + 1) return b;
+  return kNPairClass - 1;
+}
+
+// Straight-line kernel: energy units are 64-pair bra tiles x ket pairs (one
+// wavefront each, four per workgroup); J/K units are 256-pair bra strips x ket
+// pairs (one workgroup each), with the exchange rows of a unit in LDS when
+// lds_on and they fit.  per_cu = 0: as many workgroups as the kernel's occupancy
+// allows (energy: at most 2048, measured fastest for the low-AM classes).
+static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on) {
+  ClassLaunch L;
+  const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+  L.per_cu = per_cu;
+  if (nb <= 0 || nk <= 0) return L;
+  if (!jk) {
+    L.units = ((nb + kTileB - 1) / kTileB) * nk;
+    const int occ = class_occupancy(bc, kc, false, 0);
+    const int cap = per_cu > 0 ? std::min(per_cu, occ) * c->ncu : std::min(2048, occ * c->ncu);
+    L.blocks = std::min((L.units + 3) / 4, cap);
+    return L;
+  }
+  L.units = ((nb + kStripB - 1) / kStripB) * nk;
+  const size_t kb = (size_t)c->nbf * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) * sizeof(double);
+  L.lds = (lds_on && kb <= kMaxLdsK) ? kb : 0;
+  const int occ = class_occupancy(bc, kc, true, L.lds);
+  L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
+  return L;
+}
+
+// Cooperative kernel: one unit per (bra pair, ket pair); claim = 0 picks about
+// units / (4 * workgroups), between 1 and 64.
+static ClassLaunch coop_launch(vsvb_context* c, int q, bool jk, int per_cu, int claim) {
+  ClassLaunch L;
+  L.coop = true;
+  L.per_cu = per_cu;
+  const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
+  const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+  if (nb <= 0 || nk <= 0 || c->coop_idx[q] < 0) return L;  // an empty class launches nothing
+  L.units = nb * nk;
+  const int occ = coop_occupancy(c->coop_idx[q], jk);
+  L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
+  L.claim = claim > 0 ? std::min(claim, 64) : std::max(1, std::min(64, L.units / (4 * L.blocks)));
+  return L;
+}
+
 static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_context* c) {
   if (!s) return fail(VSVB_ERR_ARG, "null system");
   if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
@@ -835,38 +882,16 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   HIP_OK(c->d_tilectr.alloc(kNQuartetClass));
   HIP_OK(coop_prepare(c->device));
+  c->ncu = ncu;
   c->partial_total = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
-      const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
-      const int ci = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
-      c->coop_idx[q] = ci;
-      if (ci >= 0) {  // one wavefront (workgroup) per contracted quartet
-        const int units = (nb > 0 && nk > 0) ? nb * nk : 0;
-        c->ntiles_class[q] = units;
-        c->ntiles_jk[q] = units;
-        // an empty class launches nothing and owns no energy partials
-        c->blocks_per_class[q] = units > 0 ? std::min(units, coop_occupancy(ci, false) * ncu) : 0;
-        c->blocks_jk[q] = units > 0 ? std::min(units, coop_occupancy(ci, true) * ncu) : 0;
-        c->coop_claim[q] = units > 0 ? std::max(1, std::min(64, units / (4 * c->blocks_jk[q]))) : 1;
-        c->lds_jk[q] = 0;
-        c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;  // the kernel writes four slots per block
-        c->partial_total += c->partial_per_class[q];
-        continue;
-      }
-      const int tb = (nb + kTileB - 1) / kTileB;
-      c->ntiles_class[q] = (nb > 0 && nk > 0) ? tb * nk : 0;  // energy work units, one wavefront each
-      const int waves = (c->ntiles_class[q] + 3) / 4;
-      // Low-AM energy kernels measured fastest with at most 2048 workgroups (8192 wavefronts).
-      c->blocks_per_class[q] = std::min(waves, std::min(2048, class_occupancy(bc, kc, false, 0) * ncu));
-      // J/K: one workgroup per (256-pair bra strip, ket pair); exchange rows in LDS when they fit
-      const int nstrip = (nb + kStripB - 1) / kStripB;
-      c->ntiles_jk[q] = (nb > 0 && nk > 0) ? nstrip * nk : 0;
-      const size_t kb = (size_t)c->nbf * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) * sizeof(double);
-      c->lds_jk[q] = kb <= kMaxLdsK ? kb : 0;
-      c->blocks_jk[q] = std::min(c->ntiles_jk[q], class_occupancy(bc, kc, true, c->lds_jk[q]) * ncu);
-      c->partial_per_class[q] = (size_t)c->blocks_per_class[q] * 4;  // energy mode writes every entry
+      c->coop_idx[q] = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
+      const bool coop = coop_default(c->coop_idx[q]);
+      c->le[q] = coop ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false);
+      c->ljk[q] = coop ? coop_launch(c, q, true, 0, 0) : straight_launch(c, bc, kc, true, 0, true);
+      c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
   HIP_OK(c->d_partial.alloc(std::max<size_t>(c->partial_total, 1) * 2));
@@ -1032,6 +1057,41 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
 
 // Shell-quartet stage.  By default it serves this context's own wave function;
 // a batch group leader passes nwfn > 1 and group-wide density and J/K arrays.
+static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch& cl, bool jk, int nwfn, int nspin,
+                              const double* dens, double* jacc, double* kacc, double* partial) {
+  const int q = quartet_class_index(bc, kc);
+  QuartetArgs a{};
+  a.pairs = c->d_pairs.p;
+  a.pp = c->d_pp.p;
+  a.shells = c->d_shells.p;
+  a.boys = c->d_boys.p;
+  a.bra0 = c->pair_off[bc];
+  a.nbra = c->pair_off[bc + 1] - a.bra0;
+  a.ket0 = c->pair_off[kc];
+  a.nket = c->pair_off[kc + 1] - a.ket0;
+  a.same = bc == kc;
+  a.ntk = cl.claim;
+  a.ntiles = cl.units;
+  a.lds_k = jk && cl.lds > 0;
+  a.tau = c->tau;
+  a.nbf = c->nbf;
+  a.nwfn = nwfn;
+  a.nspin = nspin;
+  a.dens = dens;
+  a.partial = partial;
+  a.jacc = jacc;
+  a.kacc = kacc;
+  a.counters = c->d_counters.p + 4 * q;
+  a.tile_ctr = c->d_tilectr.p + q;
+  return a;
+}
+
+static hipError_t launch_class(vsvb_context* c, int bc, int kc, const ClassLaunch& cl, const QuartetArgs& a, bool jk) {
+  const int q = quartet_class_index(bc, kc);
+  if (cl.coop) return launch_coop(c->coop_idx[q], a, jk, cl.blocks, c->device, c->stream);
+  return launch_quartet_class(bc, kc, a, jk, cl.blocks, cl.lds, c->stream);
+}
+
 // nspin_override = 0 turns the J/K launch into a Coulomb-only pass over nwfn
 // general symmetric densities (the exchange accumulators are not touched).
 int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const double* dens, double* jacc,
@@ -1057,35 +1117,10 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
-      QuartetArgs a{};
-      a.pairs = c->d_pairs.p;
-      a.pp = c->d_pp.p;
-      a.shells = c->d_shells.p;
-      a.boys = c->d_boys.p;
-      a.bra0 = c->pair_off[bc];
-      a.nbra = c->pair_off[bc + 1] - a.bra0;
-      a.ket0 = c->pair_off[kc];
-      a.nket = c->pair_off[kc + 1] - a.ket0;
-      a.same = bc == kc;
-      a.ntk = c->coop_claim[q];
-      a.ntiles = jk ? c->ntiles_jk[q] : c->ntiles_class[q];
-      a.lds_k = jk && c->lds_jk[q] > 0;
-      a.tau = c->tau;
-      a.nbf = c->nbf;
-      a.nwfn = nwfn;
-      a.nspin = nspin;
-      a.dens = dens;
-      a.partial = c->d_partial.p + poff;
-      a.jacc = jacc;
-      a.kacc = kacc;
-      a.counters = c->d_counters.p + 4 * q;
-      a.tile_ctr = c->d_tilectr.p + q;
+      const ClassLaunch& cl = jk ? c->ljk[q] : c->le[q];
+      const QuartetArgs a = class_args(c, bc, kc, cl, jk, nwfn, nspin, dens, jacc, kacc, c->d_partial.p + poff);
       if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
-      if (c->coop_idx[q] >= 0)
-        HIP_OK(launch_coop(c->coop_idx[q], a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q], c->device, st));
-      else
-        HIP_OK(launch_quartet_class(bc, kc, a, jk, jk ? c->blocks_jk[q] : c->blocks_per_class[q],
-                                    jk ? c->lds_jk[q] : 0, st));
+      HIP_OK(launch_class(c, bc, kc, cl, a, jk));
       poff += c->partial_per_class[q];
     }
   if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));
@@ -1127,7 +1162,7 @@ int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
   for (int q = 0; q < kNQuartetClass; ++q) {
     for (int k = 0; k < 3; ++k) tot[k] += (double)h[4 * q + k];
     float cms = 0.0f;
-    if (c->profile && (c->last_jk ? c->ntiles_jk[q] : c->ntiles_class[q]) > 0)
+    if (c->profile && (c->last_jk ? c->ljk[q].units : c->le[q].units) > 0)
       HIP_OK(hipEventElapsedTime(&cms, c->cls_ev[q], c->cls_ev[q + 1]));
     c->class_stats[q][0] = cms;
     c->class_stats[q][1] = (double)h[4 * q];
@@ -1135,7 +1170,7 @@ int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
     c->class_stats[q][3] = (double)h[4 * q + 2];
     c->class_stats[q][4] = (double)h[4 * q + 1] * kFlopsPerPrimQuartet[q] +
                            (double)h[4 * q] * (kFlopsHrr[q] + c->last_nwfn * digestion_flops(q, nspin, c->last_jk));
-    c->class_stats[q][5] = (c->last_jk ? c->ntiles_jk[q] : c->ntiles_class[q]) > 0 ? 1.0 : 0.0;
+    c->class_stats[q][5] = (c->last_jk ? c->ljk[q].units : c->le[q].units) > 0 ? 1.0 : 0.0;
   }
   c->last_counters[0] = tot[0];
   c->last_counters[1] = tot[1];
@@ -1287,6 +1322,8 @@ struct vsvb_batch {
   std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;
   int runs = 0;
+  bool tuned = false;
+  double tuning[kNQuartetClass][6] = {};  // per class: coop, per_cu, lds, claim, best ms, default ms
   bool graph_failed = false;
   std::string graph_error;
   hipGraphExec_t exec = nullptr;
@@ -1361,6 +1398,112 @@ static int batch_gather(vsvb_batch* b) {
     HIP_OK(hipMemcpyAsync(b->d_g.p + b->goff[i], c->d_grad.p, c->totlen * sizeof(double), hipMemcpyDeviceToDevice, st));
     HIP_OK(hipMemcpyAsync(b->d_cnt.p + kCnt * i, c->d_counters.p, kCnt * sizeof(unsigned long long),
                           hipMemcpyDeviceToDevice, st));
+  }
+  return VSVB_OK;
+}
+
+// Launch tuner of the batch (J/K mode): for every class the largest closed- or
+// high-spin group leader times each candidate configuration on its own data --
+// the straight-line kernel with 1, 2, 4 or as many workgroups per CU as fit,
+// with or without the LDS exchange staging, and the cooperative kernel (classes
+// with step tables) with automatic, 1-, 8- or 64-unit claims -- and the fastest
+// (best of two launches) becomes the class policy of every wave function of the
+// batch.  The launches accumulate into the leader's J/K arrays, which the first
+// run clears again.
+struct Policy {
+  bool coop = false;
+  int per_cu = 0;
+  bool lds_on = true;
+  int claim = 0;
+};
+
+static ClassLaunch policy_launch(vsvb_context* c, int q, const Policy& p) {
+  const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
+  if (p.coop && c->coop_idx[q] >= 0) return coop_launch(c, q, true, p.per_cu, p.claim);
+  return straight_launch(c, bc, kc, true, p.per_cu, p.lds_on);
+}
+
+static int tune_batch(vsvb_batch* b) {
+  BatchGroup* gs = nullptr;
+  vsvb_context* L = nullptr;
+  for (auto& g : b->groups) {
+    vsvb_context* l = b->ctx[g.members[0]];
+    if (l->npair > 0) continue;
+    if (!L || l->nbf > L->nbf) {
+      L = l;
+      gs = &g;
+    }
+  }
+  if (!L) return VSVB_OK;
+  int rc;
+  if ((rc = prepare_geometry(L, L->coords.data()))) return rc;
+  for (int i : gs->members)
+    if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
+  const int nwfn = (int)gs->members.size();
+  const double* dens = nwfn > 1 ? gs->dens.p : L->d_R.p;
+  double* jacc = nwfn > 1 ? gs->jacc.p : L->d_J.p;
+  double* kacc = nwfn > 1 ? gs->kacc.p : L->d_K.p;
+  const int nspin = L->nunpd > 0 ? 2 : 1;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  std::vector<Policy> best(kNQuartetClass);
+  for (int bc = 0; bc < kNPairClass; ++bc)
+    for (int kc = 0; kc <= bc; ++kc) {
+      const int q = quartet_class_index(bc, kc);
+      if (L->ljk[q].units <= 0) continue;
+      std::vector<Policy> cand;
+      Policy d0;  // the default
+      d0.coop = L->ljk[q].coop;
+      cand.push_back(d0);
+      for (int pc : {0, 1, 2, 4})
+        for (bool lds : {true, false}) {
+          Policy p;
+          p.per_cu = pc;
+          p.lds_on = lds;
+          cand.push_back(p);
+        }
+      if (L->coop_idx[q] >= 0)
+        for (int cl : {0, 1, 8, 64}) {
+          Policy p;
+          p.coop = true;
+          p.claim = cl;
+          cand.push_back(p);
+        }
+      double tbest = 1e30, tdef = 0.0;
+      for (size_t k = 0; k < cand.size(); ++k) {
+        const ClassLaunch cl = policy_launch(L, q, cand[k]);
+        const QuartetArgs a = class_args(L, bc, kc, cl, true, nwfn, nspin, dens, jacc, kacc, L->d_partial.p);
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          HIP_OK(launch_zero(L->stream, L->d_tilectr.p + q, sizeof(int)));
+          HIP_OK(hipEventRecord(e0, L->stream));
+          HIP_OK(launch_class(L, bc, kc, cl, a, true));
+          HIP_OK(hipEventRecord(e1, L->stream));
+          HIP_OK(hipEventSynchronize(e1));
+          float ms = 0.0f;
+          HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+          tmin = std::min(tmin, ms);
+        }
+        if (k == 0) tdef = tmin;
+        if (tmin < tbest) {
+          tbest = tmin;
+          best[q] = cand[k];
+        }
+      }
+      b->tuning[q][0] = best[q].coop ? 1 : 0;
+      b->tuning[q][1] = best[q].per_cu;
+      b->tuning[q][2] = best[q].lds_on ? 1 : 0;
+      b->tuning[q][3] = best[q].claim;
+      b->tuning[q][4] = tbest;
+      b->tuning[q][5] = tdef;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto* c : b->ctx) {
+    if (c->npair > 0) continue;
+    for (int q = 0; q < kNQuartetClass; ++q)
+      if (b->tuning[q][4] > 0.0) c->ljk[q] = policy_launch(c, q, best[q]);
   }
   return VSVB_OK;
 }
@@ -1713,6 +1856,10 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   if (!b) return fail(VSVB_ERR_ARG, "null batch");
   int rc;
   HIP_OK(hipSetDevice(b->ctx[0]->device));
+  if (!b->tuned) {
+    b->tuned = true;
+    if ((rc = tune_batch(b))) return rc;
+  }
   ++b->runs;
   if (b->exec) return batch_replay(b, want_grad, energies, grads);
   if ((rc = batch_enqueue(b))) return rc;
@@ -1725,6 +1872,12 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   // capturing runs nothing: the graph is built now, after the first (direct) run,
   // and replayed from the next run on; on failure runs stay direct
   if (!b->graph_failed) (void)batch_capture(b);
+  return VSVB_OK;
+}
+
+int vsvb_batch_tuning(vsvb_batch* b, double* out) {
+  if (!b || !out) return fail(VSVB_ERR_ARG, "null argument");
+  std::memcpy(out, b->tuning, sizeof(b->tuning));
   return VSVB_OK;
 }
 

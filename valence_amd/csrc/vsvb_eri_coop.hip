@@ -24,6 +24,7 @@ struct CoopDesc {
   int targ;
   int nhl, hoff, hsteps;
   int out;
+  int dflt;  // used by default (the batch tuner may switch any class with tables)
 };
 
 #include "coop_tables.inc"
@@ -275,6 +276,8 @@ int coop_class(int la, int lb, int lc, int ld) {
 }
 
 size_t coop_lds_bytes(int idx) { return (size_t)kCoopDesc[idx].lds * sizeof(double); }
+
+bool coop_default(int idx) { return idx >= 0 && kCoopDesc[idx].dflt != 0; }
 
 int coop_occupancy(int idx, bool jk) {
   int nb = 0;

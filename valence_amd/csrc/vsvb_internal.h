@@ -88,6 +88,16 @@ struct SpinCoupled {
   DevBuf<double> d_q4;                            // [npairs][2][4] sign, P_R, d1, d2
 };
 
+// Launch configuration of one quartet class in one mode (energy or J/K).
+struct ClassLaunch {
+  bool coop = false;  // cooperative kernel (one wavefront per contracted quartet)
+  int units = 0;      // work units handed out by the class counter
+  int blocks = 0;     // workgroups
+  size_t lds = 0;     // straight-line J/K: exchange staging block (bytes, 0 = global atomics)
+  int claim = 1;      // cooperative kernel: units per claim
+  int per_cu = 0;     // workgroups per CU requested (0 = the kernel's occupancy)
+};
+
 }  // namespace vsvb
 
 struct vsvb_context {
@@ -129,14 +139,12 @@ struct vsvb_context {
   DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
   DevBuf<unsigned long long> d_counters;
   DevBuf<int> d_tilectr;
-  int blocks_jk[vsvb::kNQuartetClass] = {0};
+
   size_t partial_per_class[vsvb::kNQuartetClass] = {0};
-  int blocks_per_class[vsvb::kNQuartetClass] = {0};
-  int ntiles_class[vsvb::kNQuartetClass] = {0};
-  int ntiles_jk[vsvb::kNQuartetClass] = {0};      // J/K work units (256-pair bra strips x ket pairs)
-  int coop_idx[vsvb::kNQuartetClass] = {0};       // >= 0: class runs on the cooperative kernel
-  int coop_claim[vsvb::kNQuartetClass] = {0};     // cooperative kernel: units claimed per wavefront
-  size_t lds_jk[vsvb::kNQuartetClass] = {0};     // J/K exchange LDS block per workgroup (0: global atomics)
+  // launch configuration per quartet class: energy mode and J/K mode
+  vsvb::ClassLaunch le[vsvb::kNQuartetClass], ljk[vsvb::kNQuartetClass];
+  int coop_idx[vsvb::kNQuartetClass] = {0};  // cooperative step tables of the class, or -1
+  int ncu = 0;
   size_t partial_total = 0;
   bool geom_ready = false;
   double enuc = 0.0;
@@ -172,6 +180,7 @@ int check_status(vsvb_context* c, bool for_grad);
 // cooperative kernel for the register-heavy classes (vsvb_eri_coop.hip)
 int coop_class(int la, int lb, int lc, int ld);  // index of the class, or -1
 int coop_occupancy(int idx, bool jk);
+bool coop_default(int idx);
 hipError_t coop_prepare(int device);              // step tables resident on the device
 hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int device, hipStream_t st);
 

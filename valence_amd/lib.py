@@ -47,7 +47,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
-           "vsvb_version", "vsvb_batch_uses_graph"]
+           "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning"]
 
 _lib = None
 
@@ -82,6 +82,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_batch_class_stats.argtypes = [C.c_void_p, _f64p]
     L.vsvb_version.restype = C.c_char_p
     L.vsvb_batch_uses_graph.argtypes = [C.c_void_p]
+    L.vsvb_batch_tuning.argtypes = [C.c_void_p, _f64p]
     _lib = L
     return L
 
@@ -224,6 +225,12 @@ class Batch:
         if r < 0:
             _check(r, "vsvb_batch_uses_graph")
         return bool(r)
+
+    def tuning(self) -> np.ndarray:
+        """[21, 6] launch choices of the batch tuner (see vsvb_batch_tuning)."""
+        out = np.zeros((21, 6))
+        _check(load().vsvb_batch_tuning(self._h, _ptr(out, _f64p)), "vsvb_batch_tuning")
+        return out
 
     def graph_error(self) -> str:
         load().vsvb_batch_uses_graph(self._h)
