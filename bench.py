@@ -70,17 +70,27 @@ def cpu_baseline(budget_s: float):
             "energy_seconds_extrapolated": t_energy}
 
 
+def _progress(msg: str):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def tuner_summary(t) -> dict:
-    """Class -> chosen J/K launch ('coop' or 'lds'/'atomic' x workgroups per CU) and its time vs the default."""
+    """Bucket -> class -> chosen J/K launch ('coop', or 'lds'/'atomic' x workgroups per CU, 'transposed' when the
+    units are ket strips x bra pairs) and its time vs the default configuration."""
     out = {}
-    for q in range(21):
-        if t[q, 4] <= 0:
-            continue
-        if t[q, 0]:
-            kind = f"coop claim={int(t[q, 3]) or 'auto'}"
-        else:
-            kind = f"{'lds' if t[q, 2] else 'atomic'} wg/cu={int(t[q, 1]) or 'occ'}"
-        out[CLASS_NAMES[q]] = f"{kind} {t[q, 4]:.3f} ms (default {t[q, 5]:.3f})"
+    for k, bucket in enumerate(["one wave function per group", "several wave functions per group"]):
+        d = {}
+        for q in range(21):
+            r = t[k, q]
+            if r[7] <= 0:
+                continue
+            if r[0]:
+                kind = f"coop claim={int(r[3]) or 'auto'}"
+            else:
+                kind = f"{'transposed ' if r[4] else ''}{'lds' if r[2] else 'atomic'} wg/cu={int(r[1]) or 'occ'}"
+            d[CLASS_NAMES[q]] = f"{kind} {r[5]:.3f} ms (default {r[6]:.3f})"
+        if d:
+            out[bucket] = d
     return out
 
 
@@ -92,6 +102,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0, help="plain-leg steps (default: 4*steps)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="profiling aid: enqueue every batch run directly instead of replaying a HIP graph")
     ap.add_argument("--class-stats", default="", help="write per-class statistics (JSON) to this path")
     ap.add_argument("--leg", choices=["both", "plain", "batch", "copy"], default="both",
                     help="profiling aid: run one leg only (no JSON line unless both); 'copy' runs a 1 GiB "
@@ -135,6 +147,7 @@ def main():
         stats_plain += ctx.class_stats()
     t_plain = time.perf_counter() - t0
     plain_eps = nplain / t_plain if nplain else 0.0
+    _progress(f"plain leg: {nplain} energies in {t_plain:.2f} s")
     if args.leg == "plain":
         print(f"plain: {nplain} energies, {1e3 * t_plain / max(nplain, 1):.3f} ms each")
         return
@@ -142,8 +155,11 @@ def main():
     # ---- batched leg: 96 wave functions, energy + gradient
     batch = L.Batch(systems)
     batch.set_profiling(True)
+    if args.no_graph:
+        batch.set_graph(False)
     for _ in range(args.warmup):
         batch.run(want_grad=True)
+        _progress("batch leg: warm-up run done")
     stats_batch = np.zeros((21, 6))
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -151,6 +167,7 @@ def main():
         stats_batch += batch.class_stats()
     t_batch = time.perf_counter() - t0
     value = len(systems) * args.steps / t_batch
+    _progress(f"batch leg: {args.steps} runs in {t_batch:.2f} s")
     if args.class_stats:
         json.dump({"classes": CLASS_NAMES, "columns": ["ms", "quartets", "prim_quartets", "skipped", "flops", "launches"],
                    "plain_per_call": (stats_plain / max(nplain, 1)).tolist(),

@@ -121,10 +121,17 @@ void vsvb_batch_destroy(vsvb_batch *batch);
 /* 1 when runs replay the captured graph, 0 when they are enqueued directly
  * (before the first run, or after a failed capture: vsvb_last_error says why). */
 int vsvb_batch_uses_graph(vsvb_batch *batch);
-/* Launch choices of the batch tuner, which runs before the first run: for every
- * class (J/K mode), out[21][6] = {cooperative kernel 0/1, workgroups per CU
- * (0 = occupancy), LDS exchange staging 0/1, units per claim (0 = automatic),
- * best ms, default-configuration ms}; zeros for classes without work. */
+/* on = 0 keeps every run direct (an existing graph is released); on = 1 (the
+ * default) captures after the next direct run.  No counterpart in the reference.
+ * For tools that time or trace dispatches one by one and do not follow graph
+ * launches; results are the same either way. */
+int vsvb_batch_set_graph(vsvb_batch *batch, int32_t on);
+/* Launch choices of the batch tuner, which runs before the first run, for
+ * groups of one wave function (bucket 0) and groups of several sharing one
+ * quartet pass (bucket 1): out[2][21][8] = per bucket and class (J/K mode)
+ * {cooperative kernel 0/1, workgroups per CU (0 = occupancy), LDS exchange
+ * staging 0/1, units per claim (0 = automatic), transposed units 0/1, best ms,
+ * default-configuration ms, tuned 0/1}; zeros where nothing was tuned. */
 int vsvb_batch_tuning(vsvb_batch *batch, double *out);
 
 /* Diagnostics. */

@@ -172,3 +172,30 @@ def test_batch_graph_replays(lib):
             e1, g1 = ctx.energy_grad()
         assert abs(e[k] - e1) < 1e-9
         assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)
+
+
+def test_batch_graph_off_matches(lib):
+    """With graph replay switched off (before the first run, and again after a
+    graph exists) every run is direct and reproduces the replayed results."""
+    systems = workloads.config3()
+    pick = [1, 50, 92]
+    b = lib.Batch([systems[i] for i in pick])
+    e0, g0 = b.run(want_grad=True)
+    e1, g1 = b.run(want_grad=True)
+    assert b.uses_graph(), b.graph_error()
+    b.set_graph(False)
+    assert not b.uses_graph()
+    for rep in range(2):
+        e, g = b.run(want_grad=True)
+        assert not b.uses_graph()
+        for k in range(len(pick)):
+            assert abs(e[k] - e1[k]) < 1e-10, (rep, k, e[k], e1[k])
+            assert np.linalg.norm(g[k] - g1[k]) <= 1e-10 * np.linalg.norm(g1[k])
+    b2 = lib.Batch([systems[i] for i in pick])
+    b2.set_graph(False)
+    for rep in range(2):
+        e, g = b2.run(want_grad=True)
+        assert not b2.uses_graph()
+        for k in range(len(pick)):
+            assert abs(e[k] - e0[k]) < 1e-10
+            assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])

@@ -275,10 +275,14 @@ def gen_hrr(la, lb, lc, ld, accname, em: Emitter):
     return out_map
 
 
-def gen_class(la, lb, lc, ld):
+def gen_class(la, lb, lc, ld, uniform_bra=False):
+    """One class's contracted-quartet routine.  By default the ket primitive pairs
+    are wave-uniform (scalar loads) and the bra's are per lane; uniform_bra swaps
+    the two (the transposed J/K kernel: one bra pair per wavefront, ket pairs per
+    lane).  The arithmetic is the same."""
     L = la + lb + lc + ld
     em = Emitter()
-    name = f"eri_{la}{lb}{lc}{ld}"
+    name = f"eri_{la}{lb}{lc}{ld}" + ("_ub" if uniform_bra else "")
     em.emit(f"// class ({'spd'[la]}{'spd'[lb]}|{'spd'[lc]}{'spd'[ld]}), L = {L}")
     em.emit(f"__device__ __forceinline__ void {name}(const PrimPair* __restrict__ bp, int nbp,")
     em.emit(f"        const PrimPair* __restrict__ kp, int nkp, double ABx, double ABy, double ABz,")
@@ -287,13 +291,16 @@ def gen_class(la, lb, lc, ld):
     accmap, nvrr, targets = gen_vrr(la, lb, lc, ld, body)
     acc = {t: f"acc{n}" for n, t in enumerate(targets)}
     em.emit("    " + " ".join(f"double {acc[t]} = 0.0;" for t in targets))
-    em.emit("    const UniformPrimPairs kc = as_uniform(kp);  // the ket list is the same for the whole wavefront")
+    if uniform_bra:
+        em.emit("    const UniformPrimPairs bu = as_uniform(bp);  // the bra list is the same for the whole wavefront")
+    else:
+        em.emit("    const UniformPrimPairs kc = as_uniform(kp);  // the ket list is the same for the whole wavefront")
     em.emit("    for (int ib = 0; ib < nbp; ++ib) {")
-    em.emit("      const PrimPair B = bp[ib];")
+    em.emit("      const PrimPair B = " + ("load_uniform(bu + ib);" if uniform_bra else "bp[ib];"))
     if la + lb > 0:
         em.emit("      const double PAx = B.PAx, PAy = B.PAy, PAz = B.PAz, oo2p = B.oo2p;")
     em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
-    em.emit("        const PrimPair K = load_uniform(kc + ik);")
+    em.emit("        const PrimPair K = " + ("kp[ik];" if uniform_bra else "load_uniform(kc + ik);"))
     em.emit("        const double s = B.p + K.p;")
     em.emit("        const double r = rsqrt_fp64(s), rs = r * r;")
     em.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")
@@ -556,11 +563,24 @@ def main(out_path: str) -> Dict:
               "template <int LA, int LB, int LC, int LD>",
               "__device__ void eri_block(const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp,",
               "    int nkp, double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,",
+              "    const double* __restrict__ btab, double* __restrict__ out);",
+              "template <int LA, int LB, int LC, int LD>",
+              "__device__ void eri_block_ub(const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp,",
+              "    int nkp, double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,",
               "    const double* __restrict__ btab, double* __restrict__ out);", ""]
     infos = []
     for (la, lb, lc, ld) in quartet_classes():
         code, info = gen_class(la, lb, lc, ld)
         chunks.append(code)
+        if (la, lb) != (lc, ld):
+            code_ub, _ = gen_class(la, lb, lc, ld, uniform_bra=True)
+            chunks.append(code_ub)
+            chunks.append(f"template <> __device__ __forceinline__ void eri_block_ub<{la}, {lb}, {lc}, {ld}>(")
+            chunks.append("    const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp, int nkp,")
+            chunks.append("    double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
+            chunks.append("    const double* __restrict__ btab, double* __restrict__ out) {")
+            chunks.append(f"  eri_{la}{lb}{lc}{ld}_ub(bp, nbp, kp, nkp, ABx, ABy, ABz, CDx, CDy, CDz, btab, out);")
+            chunks.append("}")
         chunks.append(f"template <> __device__ __forceinline__ void eri_block<{la}, {lb}, {lc}, {ld}>(")
         chunks.append("    const PrimPair* __restrict__ bp, int nbp, const PrimPair* __restrict__ kp, int nkp,")
         chunks.append("    double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")

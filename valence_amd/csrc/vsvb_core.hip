@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -610,27 +611,27 @@ static int pair_class_of(int la, int lb) {
   return -1;
 }
 
-static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds,
-                                       hipStream_t st) {
+static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks,
+                                       size_t lds, hipStream_t st) {
   switch (bc) {
-    case 0: return launch_quartet_b0(kc, a, jk, nblocks, lds, st);
-    case 1: return launch_quartet_b1(kc, a, jk, nblocks, lds, st);
-    case 2: return launch_quartet_b2(kc, a, jk, nblocks, lds, st);
-    case 3: return launch_quartet_b3(kc, a, jk, nblocks, lds, st);
-    case 4: return launch_quartet_b4(kc, a, jk, nblocks, lds, st);
-    case 5: return launch_quartet_b5(kc, a, jk, nblocks, lds, st);
+    case 0: return launch_quartet_b0(kc, a, jk, tr, nblocks, lds, st);
+    case 1: return launch_quartet_b1(kc, a, jk, tr, nblocks, lds, st);
+    case 2: return launch_quartet_b2(kc, a, jk, tr, nblocks, lds, st);
+    case 3: return launch_quartet_b3(kc, a, jk, tr, nblocks, lds, st);
+    case 4: return launch_quartet_b4(kc, a, jk, tr, nblocks, lds, st);
+    case 5: return launch_quartet_b5(kc, a, jk, tr, nblocks, lds, st);
   }
   return hipErrorInvalidValue;
 }
 
-static int class_occupancy(int bc, int kc, bool jk, size_t lds) {
+static int class_occupancy(int bc, int kc, bool jk, bool tr, size_t lds) {
   switch (bc) {
-    case 0: return occupancy_b0(kc, jk, lds);
-    case 1: return occupancy_b1(kc, jk, lds);
-    case 2: return occupancy_b2(kc, jk, lds);
-    case 3: return occupancy_b3(kc, jk, lds);
-    case 4: return occupancy_b4(kc, jk, lds);
-    case 5: return occupancy_b5(kc, jk, lds);
+    case 0: return occupancy_b0(kc, jk, tr, lds);
+    case 1: return occupancy_b1(kc, jk, tr, lds);
+    case 2: return occupancy_b2(kc, jk, tr, lds);
+    case 3: return occupancy_b3(kc, jk, tr, lds);
+    case 4: return occupancy_b4(kc, jk, tr, lds);
+    case 5: return occupancy_b5(kc, jk, tr, lds);
   }
   return 1;
 }
@@ -659,26 +660,45 @@ static int class_bra(int q) {
 // Straight-line kernel: energy units are 64-pair bra tiles x ket pairs (one
 // wavefront each, four per workgroup); J/K units are 256-pair bra strips x ket
 // pairs (one workgroup each), with the exchange rows of a unit in LDS when
-// lds_on and they fit.  per_cu = 0: as many workgroups as the kernel's occupancy
-// allows (energy: at most 2048, measured fastest for the low-AM classes).
-static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on) {
+// lds_on and they fit, or, transposed (tr, bra and ket classes differ), ket
+// strips of 256 pairs x bra pairs.  per_cu = 0: as many workgroups as the
+// kernel's occupancy allows (energy: at most 2048, measured fastest for the
+// low-AM classes).
+static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on, bool tr) {
   ClassLaunch L;
   const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
   L.per_cu = per_cu;
   if (nb <= 0 || nk <= 0) return L;
   if (!jk) {
     L.units = ((nb + kTileB - 1) / kTileB) * nk;
-    const int occ = class_occupancy(bc, kc, false, 0);
+    const int occ = class_occupancy(bc, kc, false, false, 0);
     const int cap = per_cu > 0 ? std::min(per_cu, occ) * c->ncu : std::min(2048, occ * c->ncu);
     L.blocks = std::min((L.units + 3) / 4, cap);
     return L;
   }
-  L.units = ((nb + kStripB - 1) / kStripB) * nk;
-  const size_t kb = (size_t)c->nbf * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) * sizeof(double);
+  L.tr = tr && bc != kc;
+  const int rows = L.tr ? ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1]) : ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1]);
+  L.units = L.tr ? ((nk + kStripK - 1) / kStripK) * nb : ((nb + kStripB - 1) / kStripB) * nk;
+  const size_t kb = (size_t)c->nbf * rows * sizeof(double);
   L.lds = (lds_on && kb <= kMaxLdsK) ? kb : 0;
-  const int occ = class_occupancy(bc, kc, true, L.lds);
+  const int occ = class_occupancy(bc, kc, true, L.tr, L.lds);
   L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
   return L;
+}
+
+// Default J/K orientation of a straight-line class: transposed when its ket
+// strips keep more lanes busy than the bra strips (few bra pairs), or, at equal
+// lane use, when the bra block is the larger one (fewer per-lane atomics).
+static bool transposed_default(vsvb_context* c, int bc, int kc) {
+  if (bc == kc) return false;
+  const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+  if (nb <= 0 || nk <= 0) return false;
+  const double ub = (double)nb / (((nb + kStripB - 1) / kStripB) * kStripB);
+  const double uk = (double)nk / (((nk + kStripK - 1) / kStripK) * kStripK);
+  const int nab = ncart(kPairLs[bc][0]) * ncart(kPairLs[bc][1]), ncd = ncart(kPairLs[kc][0]) * ncart(kPairLs[kc][1]);
+  if (uk > ub + 0.05) return true;
+  if (ub > uk + 0.05) return false;
+  return nab > ncd;
 }
 
 // Cooperative kernel: one unit per (bra pair, ket pair); claim = 0 picks about
@@ -889,8 +909,9 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       const int q = quartet_class_index(bc, kc);
       c->coop_idx[q] = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
       const bool coop = coop_default(c->coop_idx[q]);
-      c->le[q] = coop ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false);
-      c->ljk[q] = coop ? coop_launch(c, q, true, 0, 0) : straight_launch(c, bc, kc, true, 0, true);
+      c->le[q] = coop ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false, false);
+      c->ljk[q] = coop ? coop_launch(c, q, true, 0, 0)
+                       : straight_launch(c, bc, kc, true, 0, true, transposed_default(c, bc, kc));
       c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
@@ -1089,7 +1110,7 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
 static hipError_t launch_class(vsvb_context* c, int bc, int kc, const ClassLaunch& cl, const QuartetArgs& a, bool jk) {
   const int q = quartet_class_index(bc, kc);
   if (cl.coop) return launch_coop(c->coop_idx[q], a, jk, cl.blocks, c->device, c->stream);
-  return launch_quartet_class(bc, kc, a, jk, cl.blocks, cl.lds, c->stream);
+  return launch_quartet_class(bc, kc, a, jk, cl.tr, cl.blocks, cl.lds, c->stream);
 }
 
 // nspin_override = 0 turns the J/K launch into a Coulomb-only pass over nwfn
@@ -1311,20 +1332,24 @@ struct BatchGroup {
   DevBuf<double> dens, jacc, kacc;
 };
 
-// The first run of a batch is enqueued eagerly; the second captures the whole
-// run (every group's geometry, orbital, quartet and gradient launches, then a
+// The first run of a batch is enqueued directly and then captured whole
+// (every group's geometry, orbital, quartet and gradient launches, then a
 // gather of all energies, gradients, status words and counters into four
-// device arrays) into one HIP graph, and later runs replay it: one graph launch
+// device arrays) into one HIP graph; later runs replay it: one graph launch
 // and one copy to pinned host memory per run.  The systems of a batch are fixed
 // at creation, so the captured work never changes.
+constexpr int kTuneBuckets = 2;  // groups of one wave function, groups of several
+constexpr int kTuneCols = 8;
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
   std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;
   int runs = 0;
   bool tuned = false;
-  double tuning[kNQuartetClass][6] = {};  // per class: coop, per_cu, lds, claim, best ms, default ms
+  // per bucket and class: coop, per_cu, lds, claim, transposed, best ms, default ms, tuned
+  double tuning[kTuneBuckets][kNQuartetClass][kTuneCols] = {};
   bool graph_failed = false;
+  bool graph_off = false;        // runs stay direct (vsvb_batch_set_graph)
   std::string graph_error;
   hipGraphExec_t exec = nullptr;
   std::vector<size_t> goff;      // offset of each wave function's gradient in d_g
@@ -1402,39 +1427,35 @@ static int batch_gather(vsvb_batch* b) {
   return VSVB_OK;
 }
 
-// Launch tuner of the batch (J/K mode): for every class the largest closed- or
-// high-spin group leader times each candidate configuration on its own data --
-// the straight-line kernel with 1, 2, 4 or as many workgroups per CU as fit,
-// with or without the LDS exchange staging, and the cooperative kernel (classes
-// with step tables) with automatic, 1-, 8- or 64-unit claims -- and the fastest
-// (best of two launches) becomes the class policy of every wave function of the
-// batch.  The launches accumulate into the leader's J/K arrays, which the first
-// run clears again.
+// Launch tuner of the batch (J/K mode), run before the first run.  Groups of
+// one wave function (one density per quartet pass) and groups of several
+// (J/K contractions of many densities per quartet) are tuned apart: for each
+// bucket a representative closed- or high-spin group leader -- single: the
+// molecule size that occurs most often; several: the group with the most
+// density work -- times every candidate configuration of every class on its
+// own data: the straight-line kernel with 1, 2, 4 or as many workgroups per CU
+// as fit, with or without the LDS exchange staging, in both orientations
+// (classes with different bra and ket classes), and the cooperative kernel
+// (classes with step tables) with automatic, 1-, 8- or 64-unit claims.  The
+// fastest (best of two launches) becomes the class policy of every group of
+// the bucket.  The launches accumulate into the leader's J/K arrays, which the
+// first run clears again.
 struct Policy {
   bool coop = false;
   int per_cu = 0;
   bool lds_on = true;
   int claim = 0;
+  bool tr = false;
 };
 
 static ClassLaunch policy_launch(vsvb_context* c, int q, const Policy& p) {
   const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
   if (p.coop && c->coop_idx[q] >= 0) return coop_launch(c, q, true, p.per_cu, p.claim);
-  return straight_launch(c, bc, kc, true, p.per_cu, p.lds_on);
+  return straight_launch(c, bc, kc, true, p.per_cu, p.lds_on, p.tr);
 }
 
-static int tune_batch(vsvb_batch* b) {
-  BatchGroup* gs = nullptr;
-  vsvb_context* L = nullptr;
-  for (auto& g : b->groups) {
-    vsvb_context* l = b->ctx[g.members[0]];
-    if (l->npair > 0) continue;
-    if (!L || l->nbf > L->nbf) {
-      L = l;
-      gs = &g;
-    }
-  }
-  if (!L) return VSVB_OK;
+static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols], std::vector<Policy>& best) {
+  vsvb_context* L = b->ctx[gs->members[0]];
   int rc;
   if ((rc = prepare_geometry(L, L->coords.data()))) return rc;
   for (int i : gs->members)
@@ -1447,7 +1468,7 @@ static int tune_batch(vsvb_batch* b) {
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
-  std::vector<Policy> best(kNQuartetClass);
+  best.assign(kNQuartetClass, Policy{});
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
@@ -1455,6 +1476,7 @@ static int tune_batch(vsvb_batch* b) {
       std::vector<Policy> cand;
       Policy d0;  // the default
       d0.coop = L->ljk[q].coop;
+      d0.tr = L->ljk[q].tr;
       cand.push_back(d0);
       for (int pc : {0, 1, 2, 4})
         for (bool lds : {true, false}) {
@@ -1462,7 +1484,17 @@ static int tune_batch(vsvb_batch* b) {
           p.per_cu = pc;
           p.lds_on = lds;
           cand.push_back(p);
+          if (bc != kc && lds) {
+            p.tr = true;
+            cand.push_back(p);
+          }
         }
+      if (bc != kc) {
+        Policy p;
+        p.tr = true;
+        p.lds_on = false;
+        cand.push_back(p);
+      }
       if (L->coop_idx[q] >= 0)
         for (int cl : {0, 1, 8, 64}) {
           Policy p;
@@ -1491,19 +1523,65 @@ static int tune_batch(vsvb_batch* b) {
           best[q] = cand[k];
         }
       }
-      b->tuning[q][0] = best[q].coop ? 1 : 0;
-      b->tuning[q][1] = best[q].per_cu;
-      b->tuning[q][2] = best[q].lds_on ? 1 : 0;
-      b->tuning[q][3] = best[q].claim;
-      b->tuning[q][4] = tbest;
-      b->tuning[q][5] = tdef;
+      double* t = tuning[q];
+      t[0] = best[q].coop ? 1 : 0;
+      t[1] = best[q].per_cu;
+      t[2] = best[q].lds_on ? 1 : 0;
+      t[3] = best[q].claim;
+      t[4] = best[q].tr ? 1 : 0;
+      t[5] = tbest;
+      t[6] = tdef;
+      t[7] = 1;
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (auto* c : b->ctx) {
-    if (c->npair > 0) continue;
-    for (int q = 0; q < kNQuartetClass; ++q)
-      if (b->tuning[q][4] > 0.0) c->ljk[q] = policy_launch(c, q, best[q]);
+  return VSVB_OK;
+}
+
+static int tune_batch(vsvb_batch* b) {
+  // representatives: single-density groups -- the most frequent molecule size
+  // (ties: the larger); multi-density groups -- the most nwfn * nbf^2
+  std::map<int, int> freq;
+  for (auto& g : b->groups) {
+    vsvb_context* l = b->ctx[g.members[0]];
+    if (l->npair == 0 && g.members.size() == 1) ++freq[l->nbf];
+  }
+  int nbf1 = -1, f1 = 0;
+  for (auto& kv : freq)
+    if (kv.second >= f1) {
+      f1 = kv.second;
+      nbf1 = kv.first;
+    }
+  BatchGroup* rep[kTuneBuckets] = {nullptr, nullptr};
+  double wmax = 0.0;
+  for (auto& g : b->groups) {
+    vsvb_context* l = b->ctx[g.members[0]];
+    if (l->npair > 0) continue;
+    if (g.members.size() == 1) {
+      if (!rep[0] && l->nbf == nbf1) rep[0] = &g;
+    } else {
+      const double w = (double)g.members.size() * l->nbf * l->nbf;
+      if (w > wmax) {
+        wmax = w;
+        rep[1] = &g;
+      }
+    }
+  }
+  std::vector<Policy> best[kTuneBuckets];
+  for (int k = 0; k < kTuneBuckets; ++k) {
+    if (!rep[k]) continue;
+    const int rc = tune_leader(b, rep[k], b->tuning[k], best[k]);
+    if (rc) return rc;
+  }
+  for (auto& g : b->groups) {
+    const int k = g.members.size() > 1 ? 1 : 0;
+    if (!rep[k]) continue;
+    for (int i : g.members) {
+      vsvb_context* c = b->ctx[i];
+      if (c->npair > 0) continue;
+      for (int q = 0; q < kNQuartetClass; ++q)
+        if (b->tuning[k][q][7] > 0.0) c->ljk[q] = policy_launch(c, q, best[k][q]);
+    }
   }
   return VSVB_OK;
 }
@@ -1871,7 +1949,18 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   }
   // capturing runs nothing: the graph is built now, after the first (direct) run,
   // and replayed from the next run on; on failure runs stay direct
-  if (!b->graph_failed) (void)batch_capture(b);
+  if (!b->graph_failed && !b->graph_off) (void)batch_capture(b);
+  return VSVB_OK;
+}
+
+int vsvb_batch_set_graph(vsvb_batch* b, int32_t on) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  b->graph_off = on == 0;
+  if (b->graph_off && b->exec) {
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
   return VSVB_OK;
 }
 

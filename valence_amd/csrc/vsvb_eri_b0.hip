@@ -3,16 +3,16 @@
 
 namespace vsvb {
 
-hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st) {
+hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st) {
   switch (kc) {
-    case 0: return launch_q<0, 0, 0, 0>(a, jk, nblocks, lds, st);
+    case 0: return launch_q<0, 0, 0, 0>(a, jk, tr, nblocks, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-int occupancy_b0(int kc, bool jk, size_t lds) {
+int occupancy_b0(int kc, bool jk, bool tr, size_t lds) {
   switch (kc) {
-    case 0: return occupancy_q<0, 0, 0, 0>(jk, lds);
+    case 0: return occupancy_q<0, 0, 0, 0>(jk, tr, lds);
     default: return 1;
   }
 }

@@ -3,26 +3,26 @@
 
 namespace vsvb {
 
-hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st) {
+hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st) {
   switch (kc) {
-    case 0: return launch_q<2, 2, 0, 0>(a, jk, nblocks, lds, st);
-    case 1: return launch_q<2, 2, 1, 0>(a, jk, nblocks, lds, st);
-    case 2: return launch_q<2, 2, 1, 1>(a, jk, nblocks, lds, st);
-    case 3: return launch_q<2, 2, 2, 0>(a, jk, nblocks, lds, st);
-    case 4: return launch_q<2, 2, 2, 1>(a, jk, nblocks, lds, st);
-    case 5: return launch_q<2, 2, 2, 2>(a, jk, nblocks, lds, st);
+    case 0: return launch_q<2, 2, 0, 0>(a, jk, tr, nblocks, lds, st);
+    case 1: return launch_q<2, 2, 1, 0>(a, jk, tr, nblocks, lds, st);
+    case 2: return launch_q<2, 2, 1, 1>(a, jk, tr, nblocks, lds, st);
+    case 3: return launch_q<2, 2, 2, 0>(a, jk, tr, nblocks, lds, st);
+    case 4: return launch_q<2, 2, 2, 1>(a, jk, tr, nblocks, lds, st);
+    case 5: return launch_q<2, 2, 2, 2>(a, jk, tr, nblocks, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-int occupancy_b5(int kc, bool jk, size_t lds) {
+int occupancy_b5(int kc, bool jk, bool tr, size_t lds) {
   switch (kc) {
-    case 0: return occupancy_q<2, 2, 0, 0>(jk, lds);
-    case 1: return occupancy_q<2, 2, 1, 0>(jk, lds);
-    case 2: return occupancy_q<2, 2, 1, 1>(jk, lds);
-    case 3: return occupancy_q<2, 2, 2, 0>(jk, lds);
-    case 4: return occupancy_q<2, 2, 2, 1>(jk, lds);
-    case 5: return occupancy_q<2, 2, 2, 2>(jk, lds);
+    case 0: return occupancy_q<2, 2, 0, 0>(jk, tr, lds);
+    case 1: return occupancy_q<2, 2, 1, 0>(jk, tr, lds);
+    case 2: return occupancy_q<2, 2, 1, 1>(jk, tr, lds);
+    case 3: return occupancy_q<2, 2, 2, 0>(jk, tr, lds);
+    case 4: return occupancy_q<2, 2, 2, 1>(jk, tr, lds);
+    case 5: return occupancy_q<2, 2, 2, 2>(jk, tr, lds);
     default: return 1;
   }
 }

@@ -140,8 +140,8 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
     if (!__any(act)) continue;  // wave-uniform: nothing to contract
     const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
                            : 0.0;
-    // inactive lanes read a valid (ket) block and contribute zero
-    const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
+    // inactive lanes read valid blocks and contribute zero
+    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
     const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
     for (int w = 0; w < a.nwfn; ++w) {
       const double* D0 = a.dens + (size_t)w * (a.nspin > 0 ? a.nspin : 1) * nn;
@@ -261,13 +261,12 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
     const int ib = st * kStripB + wave * kTileB + lane;
     const Pair K = a.pairs[a.ket0 + ik];
     bool act = ib < a.nbra && (!a.same || ik <= ib);
-    Pair P{};
-    if (act) {
-      P = a.pairs[a.bra0 + ib];
-      if (P.Q * K.Q < a.tau) {
-        act = false;
-        ++nskip;
-      }
+    // every lane holds a pair of the bra class (the last one past the end), so
+    // that the density blocks read below stay inside the matrix
+    const Pair P = a.pairs[a.bra0 + (ib < a.nbra ? ib : a.nbra - 1)];
+    if (act && P.Q * K.Q < a.tau) {
+      act = false;
+      ++nskip;
     }
     double v[NAB * NCD];
     if (act) {
@@ -412,6 +411,198 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
   }
 }
 
+// Transposed J/K unit, for classes whose bra and ket pair classes differ: one
+// bra pair for the whole workgroup and a strip of 256 ket pairs, one per lane.
+// Units are ket strips x bra pairs (bra pair fastest).  The bra pair's
+// primitive data are scalar loads; the exchange rows of the bra's AOs (NA + NB
+// rows of nbf) are accumulated in LDS and flushed once per unit; the Coulomb
+// block of the bra is a wave sum, those of the kets per-lane atomics.  Suits
+// classes with few bra pairs (a 256-pair bra strip would leave most lanes
+// idle) or large bra blocks (fewer global atomics per quartet).
+constexpr int kStripK = 256;
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NR = NA + NB;
+  extern __shared__ double kbuf[];
+  __shared__ int s_unit;
+  const int n = a.nbf;
+  const size_t nn = (size_t)n * n;
+  const int ns = a.nspin;
+  const int dstride = ns > 0 ? ns : 1;
+  const double cfac = ns == 1 ? 2.0 : 1.0;
+  const double xfac = ns == 1 ? 2.0 : 1.0;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool lds = a.lds_k != 0;
+  const int kb_len = lds ? n * NR : 0;
+  for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
+  unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  for (;;) {
+    __syncthreads();  // the previous unit is flushed and s_unit is free
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr, 1);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (u >= a.ntiles) break;
+    const int st = u / a.nbra;
+    const int ib = __builtin_amdgcn_readfirstlane(u - st * a.nbra);
+    const int ik = st * kStripK + wave * kTileB + lane;
+    const Pair P = a.pairs[a.bra0 + ib];
+    bool act = ik < a.nket;
+    // every lane holds a pair of the ket class (the last one past the end)
+    const Pair K = a.pairs[a.ket0 + (act ? ik : a.nket - 1)];
+    if (act && P.Q * K.Q < a.tau) {
+      act = false;
+      ++nskip;
+    }
+    double v[NAB * NCD];
+    if (act) {
+      eri_block_ub<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                   K.ABz, a.boys, v);
+      ++nq;
+      nprimq += (unsigned long long)P.npp * K.npp;
+    } else {
+#pragma unroll
+      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
+    }
+    const bool wact = __any(act);  // wave-uniform
+    // bra and ket classes differ, so every quartet stands for its (cd|ab) image too
+    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * 2.0 : 0.0;
+    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
+    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+    const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+    for (int w = 0; w < a.nwfn; ++w) {
+      const double* D0 = a.dens + (size_t)w * dstride * nn;
+      if (wact) {
+        double rab[NAB], rcd[NCD];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            double t = D0[(size_t)(ma + i) * n + mb + j];
+            if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
+            rab[i * NB + j] = cfac * t;
+          }
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) {
+            double t = D0[(size_t)(mc + k) * n + md + l];
+            if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
+            rcd[k * ND + l] = cfac * t;
+          }
+        double* J = a.jacc + (size_t)w * nn;
+        if (act) {
+#pragma unroll
+          for (int kl = 0; kl < NCD; ++kl) {
+            double t = 0.0;
+#pragma unroll
+            for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
+            unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], fj * t);
+          }
+        }
+#pragma unroll
+        for (int ij = 0; ij < NAB; ++ij) {
+          double t = 0.0;
+#pragma unroll
+          for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+          t = wave_sum(fj * t);
+          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], t);
+        }
+      }
+      for (int s = 0; s < ns; ++s) {
+        double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+        if (act) {
+          DBlocks<NA, NB, NC, ND> d;
+          d.load(D0 + s * nn, n, ma, mb, mc, md);
+          // target (bra row r < NR, column col): LDS block or the global accumulator
+          auto put = [&](int r, int col, double x) {
+            if (lds)
+              unsafeAtomicAdd(&kbuf[r * n + col], x);
+            else
+              unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
+          };
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+              double t = 0.0;
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
+              put(i, mc + k, fk * t);
+            }
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
+              put(NA + j, mc + k, fk * t);
+            }
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+              double t = 0.0;
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
+              put(i, md + l, fk * t);
+            }
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
+              put(NA + j, md + l, fk * t);
+            }
+        }
+        if (lds) {
+          __syncthreads();
+          for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
+            const double x = kbuf[e];
+            if (x != 0.0) {
+              kbuf[e] = 0.0;
+              const int r = e / n, col = e - r * n;
+              unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (a.counters) {
+    unsigned long long q = nq, pq = nprimq, sk = nskip;
+    for (int o = 32; o > 0; o >>= 1) {
+      q += __shfl_xor(q, o, 64);
+      pq += __shfl_xor(pq, o, 64);
+      sk += __shfl_xor(sk, o, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], q);
+      atomicAdd(&a.counters[1], pq);
+      atomicAdd(&a.counters[2], sk);
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) quartet_jkt_kernel(QuartetArgs a) {
+  quartet_jkt_body<LA, LB, LC, LD>(a);
+}
+
 template <int LA, int LB, int LC, int LD, bool kJK>
 __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
   if constexpr (kJK)
@@ -448,8 +639,14 @@ __global__ void __launch_bounds__(256) schwarz_kernel(Pair* pairs, const PrimPai
 }
 
 template <int LA, int LB, int LC, int LD>
-static int occupancy_q(bool jk, size_t lds_bytes) {
+static int occupancy_q(bool jk, bool tr, size_t lds_bytes) {
   int n = 0;
+  if constexpr (LA != LC || LB != LD) {
+    if (jk && tr) {
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_jkt_kernel<LA, LB, LC, LD>, 256, lds_bytes);
+      return n > 0 ? n : 1;
+    }
+  }
   if (jk)
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, quartet_kernel<LA, LB, LC, LD, true>, 256, lds_bytes);
   else
@@ -458,8 +655,16 @@ static int occupancy_q(bool jk, size_t lds_bytes) {
 }
 
 template <int LA, int LB, int LC, int LD>
-static hipError_t launch_q(const QuartetArgs& a, bool jk, int nblocks, size_t lds_bytes, hipStream_t st) {
+static hipError_t launch_q(const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds_bytes, hipStream_t st) {
   if (nblocks <= 0 || a.ntiles <= 0) return hipSuccess;
+  if constexpr (LA != LC || LB != LD) {
+    if (jk && tr) {
+      hipLaunchKernelGGL((quartet_jkt_kernel<LA, LB, LC, LD>), dim3(nblocks), dim3(256), lds_bytes, st, a);
+      return hipGetLastError();
+    }
+  } else {
+    if (tr) return hipErrorInvalidValue;  // no transposed kernel for a class paired with itself
+  }
   if (jk)
     hipLaunchKernelGGL((quartet_kernel<LA, LB, LC, LD, true>), dim3(nblocks), dim3(256), lds_bytes, st, a);
   else
@@ -468,18 +673,18 @@ static hipError_t launch_q(const QuartetArgs& a, bool jk, int nblocks, size_t ld
 }
 
 // Per-bra-class launchers (one translation unit each).
-hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_quartet_b1(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, int nblocks, size_t lds, hipStream_t st);
-int occupancy_b0(int kc, bool jk, size_t lds);
-int occupancy_b1(int kc, bool jk, size_t lds);
-int occupancy_b2(int kc, bool jk, size_t lds);
-int occupancy_b3(int kc, bool jk, size_t lds);
-int occupancy_b4(int kc, bool jk, size_t lds);
-int occupancy_b5(int kc, bool jk, size_t lds);
+hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b1(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+int occupancy_b0(int kc, bool jk, bool tr, size_t lds);
+int occupancy_b1(int kc, bool jk, bool tr, size_t lds);
+int occupancy_b2(int kc, bool jk, bool tr, size_t lds);
+int occupancy_b3(int kc, bool jk, bool tr, size_t lds);
+int occupancy_b4(int kc, bool jk, bool tr, size_t lds);
+int occupancy_b5(int kc, bool jk, bool tr, size_t lds);
 hipError_t launch_schwarz_b0(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b1(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b2(Pair*, const PrimPair*, const double*, int, int, hipStream_t);

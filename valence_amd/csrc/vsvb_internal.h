@@ -96,6 +96,7 @@ struct ClassLaunch {
   size_t lds = 0;     // straight-line J/K: exchange staging block (bytes, 0 = global atomics)
   int claim = 1;      // cooperative kernel: units per claim
   int per_cu = 0;     // workgroups per CU requested (0 = the kernel's occupancy)
+  bool tr = false;    // straight-line J/K: transposed units (one bra pair x 256 ket pairs)
 };
 
 }  // namespace vsvb

@@ -47,7 +47,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
-           "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning"]
+           "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph"]
 
 _lib = None
 
@@ -83,6 +83,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_version.restype = C.c_char_p
     L.vsvb_batch_uses_graph.argtypes = [C.c_void_p]
     L.vsvb_batch_tuning.argtypes = [C.c_void_p, _f64p]
+    L.vsvb_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
     _lib = L
     return L
 
@@ -219,6 +220,10 @@ class Batch:
     def set_profiling(self, on: bool = True):
         _check(load().vsvb_batch_set_profiling(self._h, int(on)), "vsvb_batch_set_profiling")
 
+    def set_graph(self, on: bool = True):
+        """Off: every run is enqueued directly (no graph capture or replay)."""
+        _check(load().vsvb_batch_set_graph(self._h, int(on)), "vsvb_batch_set_graph")
+
     def uses_graph(self) -> bool:
         """True once runs replay the captured HIP graph (captured after the first run)."""
         r = load().vsvb_batch_uses_graph(self._h)
@@ -227,8 +232,8 @@ class Batch:
         return bool(r)
 
     def tuning(self) -> np.ndarray:
-        """[21, 6] launch choices of the batch tuner (see vsvb_batch_tuning)."""
-        out = np.zeros((21, 6))
+        """[2, 21, 8] launch choices of the batch tuner (see vsvb_batch_tuning)."""
+        out = np.zeros((2, 21, 8))
         _check(load().vsvb_batch_tuning(self._h, _ptr(out, _f64p)), "vsvb_batch_tuning")
         return out
 
