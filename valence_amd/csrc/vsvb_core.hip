@@ -159,7 +159,7 @@ __global__ void k_prim_screen(Pair* __restrict__ pairs, int np, const Shell* __r
 }
 
 // ---------------------------------------------------------------------------
-// device: one-electron integrals (Obara-Saika), one thread per shell pair
+// device: one-electron integrals (Obara-Saika), one wavefront per shell pair
 // ---------------------------------------------------------------------------
 // cartesian tuples of total order 0..4 in CCA order, cumulative index
 __device__ __forceinline__ constexpr int cart_off(int l) { return l * (l + 1) * (l + 2) / 6; }
@@ -175,18 +175,22 @@ __device__ __forceinline__ void cart_of(int l, int c, int& lx, int& ly, int& lz)
   lz = j;
 }
 
+// The lanes of the wavefront take the (primitive pair, nuclear centre) items of
+// the pair in turn; the item of centre 0 also adds the primitive pair's overlap
+// and kinetic terms.  Lane sums are reduced across the wavefront at the end.
 template <int LA, int LB>
-__global__ void __launch_bounds__(64) k_one_e(const Pair* __restrict__ pairs, int p0, int np,
-                                              const Shell* __restrict__ shells, const double* __restrict__ pexp,
-                                              const double* __restrict__ pcoef, const double* __restrict__ xyz,
-                                              const double* __restrict__ charge, int natom,
-                                              const double* __restrict__ boys, double* __restrict__ S,
-                                              double* __restrict__ H, int nbf) {
+__global__ void __launch_bounds__(256) k_one_e(const Pair* __restrict__ pairs, int p0, int np,
+                                               const Shell* __restrict__ shells, const double* __restrict__ pexp,
+                                               const double* __restrict__ pcoef, const double* __restrict__ xyz,
+                                               const double* __restrict__ charge, int natom,
+                                               const double* __restrict__ boys, double* __restrict__ S,
+                                               double* __restrict__ H, int nbf) {
   constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
   constexpr int NE = cart_off(L + 1);        // cartesians of order 0..L
   constexpr int NBc = cart_off(LB + 1);      // cartesians of order 0..LB
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
+  const int t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  if (t >= np) return;  // whole wavefront
   const Pair PR = pairs[p0 + t];
   const Shell A = shells[PR.A], B = shells[PR.B];
   const double Ax = xyz[3 * A.atom], Ay = xyz[3 * A.atom + 1], Az = xyz[3 * A.atom + 2];
@@ -195,14 +199,18 @@ __global__ void __launch_bounds__(64) k_one_e(const Pair* __restrict__ pairs, in
   double s[NA * NB], tk[NA * NB], v[NA * NB];
 #pragma unroll
   for (int k = 0; k < NA * NB; ++k) s[k] = tk[k] = v[k] = 0.0;
-  for (int ia = 0; ia < A.nprim; ++ia)
-    for (int ib = 0; ib < B.nprim; ++ib) {
+  const int nitem = A.nprim * B.nprim * natom;
+  for (int item = lane; item < nitem; item += 64) {
+    const int ipp = item / natom, c = item - ipp * natom;
+    const int ia = ipp / B.nprim, ib = ipp - ia * B.nprim;
+    {
       const double a = pexp[A.prim0 + ia], b = pexp[B.prim0 + ib], p = a + b, rp = 1.0 / p;
       const double mu = a * b * rp;
       const double cc = pcoef[A.prim0 + ia] * pcoef[B.prim0 + ib];
       const double Pc[3] = {(a * Ax + b * Bx) * rp, (a * Ay + b * By) * rp, (a * Az + b * Bz) * rp};
       const double PA[3] = {Pc[0] - Ax, Pc[1] - Ay, Pc[2] - Az};
       const double PB[3] = {Pc[0] - Bx, Pc[1] - By, Pc[2] - Bz};
+      if (c == 0) {
       // 1D overlaps, i <= LA, j <= LB + 2
       double S1[3][LA + 1][LB + 3];
 #pragma unroll
@@ -244,9 +252,10 @@ __global__ void __launch_bounds__(64) k_one_e(const Pair* __restrict__ pairs, in
           s[ca * NB + cb] += cc * sv[0] * sv[1] * sv[2];
           tk[ca * NB + cb] += cc * (tv[0] * sv[1] * sv[2] + sv[0] * tv[1] * sv[2] + sv[0] * sv[1] * tv[2]);
         }
-      // nuclear attraction: VRR [e|0]^(m), |e| <= L, then HRR to [a|b]
+      }
+      // nuclear attraction of centre c: VRR [e|0]^(m), |e| <= L, then HRR to [a|b]
       const double pre = 2.0 * M_PI * rp * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2])) * cc;
-      for (int c = 0; c < natom; ++c) {
+      {
         const double Z = charge[c];
         if (fabs(Z) <= 1e-12) continue;  // int1e skips |Z| <= 1e-12 (valence.F90:3146)
         const double PC[3] = {Pc[0] - xyz[3 * c], Pc[1] - xyz[3 * c + 1], Pc[2] - xyz[3 * c + 2]};
@@ -316,16 +325,19 @@ __global__ void __launch_bounds__(64) k_one_e(const Pair* __restrict__ pairs, in
           for (int cb = 0; cb < NB; ++cb) v[ca * NB + cb] += W[cart_off(LA) + ca][cart_off(LB) + cb];
       }
     }
+  }
 #pragma unroll
   for (int ca = 0; ca < NA; ++ca)
 #pragma unroll
     for (int cb = 0; cb < NB; ++cb) {
-      const int mu = A.ao + ca, nu = B.ao + cb;
-      const double sv = s[ca * NB + cb], hv = tk[ca * NB + cb] + v[ca * NB + cb];
-      S[(size_t)mu * nbf + nu] = sv;
-      S[(size_t)nu * nbf + mu] = sv;
-      H[(size_t)mu * nbf + nu] = hv;
-      H[(size_t)nu * nbf + mu] = hv;
+      const double sv = vsvb::wave_sum(s[ca * NB + cb]), hv = vsvb::wave_sum(tk[ca * NB + cb] + v[ca * NB + cb]);
+      if (lane == 0) {
+        const int mu = A.ao + ca, nu = B.ao + cb;
+        S[(size_t)mu * nbf + nu] = sv;
+        S[(size_t)nu * nbf + mu] = sv;
+        H[(size_t)mu * nbf + nu] = hv;
+        H[(size_t)nu * nbf + mu] = hv;
+      }
     }
 }
 
@@ -333,7 +345,7 @@ static hipError_t launch_one_e(int pc, const Pair* pairs, int p0, int np, const 
                                const double* pcoef, const double* xyz, const double* charge, int natom,
                                const double* boys, double* S, double* H, int nbf, hipStream_t st) {
   if (np <= 0) return hipSuccess;
-  const dim3 g((np + 63) / 64), b(64);
+  const dim3 g((np + 3) / 4), b(256);  // one wavefront per shell pair
 #define LAUNCH(LA, LB) \
   hipLaunchKernelGGL((k_one_e<LA, LB>), g, b, 0, st, pairs, p0, np, shells, pexp, pcoef, xyz, charge, natom, boys, S, H, nbf)
   switch (pc) {
@@ -1340,10 +1352,15 @@ struct BatchGroup {
 // at creation, so the captured work never changes.
 constexpr int kTuneBuckets = 2;  // groups of one wave function, groups of several
 constexpr int kTuneCols = 8;
+constexpr int kBatchLanes = 4;  // streams the groups of a batch run on, round-robin
+
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
   std::vector<BatchGroup> groups;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;          // the run's own stream: fork, join, gather, graph launch
+  std::vector<hipStream_t> lanes;        // group streams (default priority), forked from and joined to stream
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
   int runs = 0;
   bool tuned = false;
   // per bucket and class: coop, per_cu, lds, claim, transposed, best ms, default ms, tuned
@@ -1369,15 +1386,22 @@ struct vsvb_batch {
     if (h_cnt) (void)hipHostFree(h_cnt);
     groups.clear();
     for (auto* c : ctx) vsvb_context_destroy(c);
+    for (auto e : ev_join) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (auto s : lanes) (void)hipStreamDestroy(s);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 constexpr int kCnt = 4 * kNQuartetClass;  // counters per context
 
-// All device work of one batch run, no host synchronization.
+// All device work of one batch run, no host synchronization.  The groups run
+// on the lane streams, which fork from the batch stream at the start and join
+// it at the end (in a captured graph: parallel branches).
 static int batch_enqueue(vsvb_batch* b) {
   int rc;
+  HIP_OK(hipEventRecord(b->ev_fork, b->stream));
+  for (auto s : b->lanes) HIP_OK(hipStreamWaitEvent(s, b->ev_fork, 0));
   // Every wave function goes through the J/K (adjoint-ready) contraction; the
   // energy is assembled from it, and the gradient when asked for.
   for (auto& g : b->groups) {
@@ -1404,6 +1428,10 @@ static int batch_enqueue(vsvb_batch* b) {
       m->enuc = lead->enuc;
       if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
     }
+  }
+  for (size_t k = 0; k < b->lanes.size(); ++k) {
+    HIP_OK(hipEventRecord(b->ev_join[k], b->lanes[k]));
+    HIP_OK(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
   }
   return VSVB_OK;
 }
@@ -1897,6 +1925,19 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
         bt->groups.back().members.push_back(j);
       }
   }
+  // group streams: group k runs on lane k mod kBatchLanes
+  const int nl = std::min<int>(kBatchLanes, (int)bt->groups.size());
+  HIP_OK(hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming));
+  for (int k = 0; k < nl; ++k) {
+    hipStream_t s = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    bt->lanes.push_back(s);
+    hipEvent_t e = nullptr;
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    bt->ev_join.push_back(e);
+  }
+  for (size_t k = 0; k < bt->groups.size(); ++k)
+    for (int i : bt->groups[k].members) bt->ctx[i]->stream = bt->lanes[k % nl];
   for (auto& g : bt->groups) {
     vsvb_context* lead = bt->ctx[g.members[0]];
     if (g.members.size() < 2) continue;
