@@ -167,6 +167,8 @@ def main():
         stats_batch += batch.class_stats()
     t_batch = time.perf_counter() - t0
     value = len(systems) * args.steps / t_batch
+    uses_graph = batch.uses_graph()
+    tuning = batch.tuning()
     _progress(f"batch leg: {args.steps} runs in {t_batch:.2f} s")
     if args.class_stats:
         json.dump({"classes": CLASS_NAMES, "columns": ["ms", "quartets", "prim_quartets", "skipped", "flops", "launches"],
@@ -178,12 +180,23 @@ def main():
 
     # ---- roofline of the dominant kernel of the measured (batched) workload: the
     # J/K instantiation of the class with the largest summed device time, timed by
-    # HIP events on the library stream around each class launch
+    # HIP events on the stream of each class launch over the timed region.  The
+    # groups overlap there (four streams), so a launch shares the GPU with
+    # others; the same kernel is also timed alone, in one more (untimed) run
+    # with the groups on one stream and no graph.
     q = int(np.argmax(stats_batch[:, 0]))
     nlaunch = int(round(stats_batch[q, 5]))
     avg_ms = stats_batch[q, 0] / max(nlaunch, 1)
     flops_per_launch = stats_batch[q, 4] / max(nlaunch, 1)
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    batch.set_graph(False)
+    batch.set_lanes(1)
+    batch.run(want_grad=True)
+    iso = batch.class_stats()
+    iso_ms = iso[q, 0] / max(iso[q, 5], 1)
+    iso_flops = iso[q, 4] / max(iso[q, 5], 1)
+    _progress("batch leg: isolated-kernel run done")
+    quartet_tflops_step = stats_batch[:, 4].sum() / args.steps / (t_batch / args.steps) / 1e12
     kname = f"quartet_kernel<{', '.join(CLASS_KERNELS[q])}, true>"
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_quartet_jk_c8h18_batch96.json")
@@ -218,8 +231,10 @@ def main():
             "plain_energy_Eh": e_plain,
             "plain_eri_stage_ms": eri_ms_plain,
             "plain_eri_stage_tflops": flops_plain / (eri_ms_plain * 1e-3) / 1e12 if eri_ms_plain > 0 else None,
-            "batch_runs_replay_hip_graph": batch.uses_graph(),
-            "batch_tuner": tuner_summary(batch.tuning()),
+            "batch_runs_replay_hip_graph": uses_graph,
+            "batch_group_streams": 4,
+            "batch_quartet_stage_tflops_over_step": quartet_tflops_step,
+            "batch_tuner": tuner_summary(tuning),
         },
         "roofline": {
             "bound": "mfma",
@@ -234,6 +249,17 @@ def main():
             "avg_launch_ms": avg_ms,
             "launches": nlaunch,
             "flops_per_launch": flops_per_launch,
+            "note": "launches of the timed region overlap those of other groups (four streams)",
+        },
+        "roofline_isolated": {
+            "kernel": f"{kname} ({CLASS_NAMES[q]}, J/K mode)",
+            "achieved": iso_flops / (iso_ms * 1e-3) / 1e12 if iso_ms > 0 else None,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": iso_flops / (iso_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS if iso_ms > 0 else None,
+            "avg_launch_ms": iso_ms,
+            "launches": int(round(iso[q, 5])),
+            "note": "one untimed batch run after the timed region, groups on one stream, no graph",
         },
     }
     if not args.no_cpu_baseline:

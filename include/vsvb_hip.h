@@ -109,9 +109,10 @@ int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, doubl
 /* Batched evaluation of many wave functions of uneven size (displaced
  * geometries, trial orbital rotations, different molecules); no counterpart in
  * the reference, which evaluates one wave function per process call.  Wave
- * functions on one molecule share one pass over the shell quartets.  The first
- * run is enqueued directly and then captured into a HIP graph; every later run
- * is one graph launch (the systems are fixed at creation). */
+ * functions on one molecule share one pass over the shell quartets; the groups
+ * run on several streams so that they overlap.  The first run is enqueued
+ * directly and then captured into a HIP graph; every later run is one graph
+ * launch (the systems are fixed at creation). */
 typedef struct vsvb_batch vsvb_batch;
 
 int vsvb_batch_create(int32_t nwfn, const vsvb_system *const *systems, const vsvb_options *opt, vsvb_batch **batch);
@@ -126,6 +127,12 @@ int vsvb_batch_uses_graph(vsvb_batch *batch);
  * For tools that time or trace dispatches one by one and do not follow graph
  * launches; results are the same either way. */
 int vsvb_batch_set_graph(vsvb_batch *batch, int32_t on);
+/* Groups of a batch run on n streams (1..4, default 4; more is clipped), group
+ * k on stream k mod n, so that independent groups overlap on the GPU.  An
+ * existing graph is released (the next run captures again unless graphs are
+ * off).  n = 1 serializes the groups, for per-kernel timing without overlap.
+ * No counterpart in the reference. */
+int vsvb_batch_set_lanes(vsvb_batch *batch, int32_t n);
 /* Launch choices of the batch tuner, which runs before the first run, for
  * groups of one wave function (bucket 0) and groups of several sharing one
  * quartet pass (bucket 1): out[2][21][8] = per bucket and class (J/K mode)

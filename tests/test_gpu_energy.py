@@ -199,3 +199,19 @@ def test_batch_graph_off_matches(lib):
         for k in range(len(pick)):
             assert abs(e[k] - e0[k]) < 1e-10
             assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])
+
+
+def test_batch_lanes_match(lib):
+    """Groups overlap on several streams by default; one or two streams must give
+    the same results (each group writes only its own buffers)."""
+    systems = workloads.config3()
+    pick = [2, 3, 51, 52, 93, 94]
+    b = lib.Batch([systems[i] for i in pick])
+    e0, g0 = b.run(want_grad=True)
+    for n in (1, 2):
+        b.set_lanes(n)
+        for rep in range(2):                       # direct run, then a replay of the recaptured graph
+            e, g = b.run(want_grad=True)
+            for k in range(len(pick)):
+                assert abs(e[k] - e0[k]) < 1e-10, (n, rep, k, e[k], e0[k])
+                assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])

@@ -1994,6 +1994,21 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   return VSVB_OK;
 }
 
+int vsvb_batch_set_lanes(vsvb_batch* b, int32_t n) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  if (n < 1) return fail(VSVB_ERR_ARG, "lane count must be at least 1");
+  const int nl = std::min<int>(n, (int)b->lanes.size());
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (auto s : b->lanes) HIP_OK(hipStreamSynchronize(s));
+  if (b->exec) {
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  for (size_t k = 0; k < b->groups.size(); ++k)
+    for (int i : b->groups[k].members) b->ctx[i]->stream = b->lanes[k % nl];
+  return VSVB_OK;
+}
+
 int vsvb_batch_set_graph(vsvb_batch* b, int32_t on) {
   if (!b) return fail(VSVB_ERR_ARG, "null batch");
   b->graph_off = on == 0;

@@ -47,7 +47,8 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_energy_grad_device", "vsvb_batch_create", "vsvb_batch_run", "vsvb_batch_destroy",
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
-           "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph"]
+           "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
+           "vsvb_batch_set_lanes"]
 
 _lib = None
 
@@ -84,6 +85,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_batch_uses_graph.argtypes = [C.c_void_p]
     L.vsvb_batch_tuning.argtypes = [C.c_void_p, _f64p]
     L.vsvb_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_batch_set_lanes.argtypes = [C.c_void_p, C.c_int32]
     _lib = L
     return L
 
@@ -223,6 +225,10 @@ class Batch:
     def set_graph(self, on: bool = True):
         """Off: every run is enqueued directly (no graph capture or replay)."""
         _check(load().vsvb_batch_set_graph(self._h, int(on)), "vsvb_batch_set_graph")
+
+    def set_lanes(self, n: int):
+        """Number of streams the groups run on (1 serializes them)."""
+        _check(load().vsvb_batch_set_lanes(self._h, int(n)), "vsvb_batch_set_lanes")
 
     def uses_graph(self) -> bool:
         """True once runs replay the captured HIP graph (captured after the first run)."""
