@@ -197,7 +197,13 @@ def main():
     iso_flops = iso[q, 4] / max(iso[q, 5], 1)
     _progress("batch leg: isolated-kernel run done")
     quartet_tflops_step = stats_batch[:, 4].sum() / args.steps / (t_batch / args.steps) / 1e12
-    kname = f"quartet_kernel<{', '.join(CLASS_KERNELS[q])}, true>"
+    # the kernel serving the class in the groups of one wave function (most launches)
+    if tuning[0, q, 0]:
+        kname = "coop_kernel<true>"
+    elif tuning[0, q, 4]:
+        kname = f"quartet_jkt_kernel<{', '.join(CLASS_KERNELS[q])}>"
+    else:
+        kname = f"quartet_kernel<{', '.join(CLASS_KERNELS[q])}, true>"
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_quartet_jk_c8h18_batch96.json")
     if os.path.exists(tpath):
