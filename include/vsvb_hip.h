@@ -163,6 +163,11 @@ int vsvb_debug_buffer(vsvb_context *ctx, const char *name, double *out, int64_t 
 /* Evaluate the device Boys function F_0..F_L at n arguments T (diagnostics/tests):
  * out[i*(L+1) + m] = F_m(T[i]), 0 <= L <= 8. */
 int vsvb_debug_boys(int32_t L, const double *T, int32_t n, double *out);
+/* The library's dense product on the device (diagnostics/tests), row-major:
+ * C[M x N] = alpha op(A) op(B) + beta C, op(A) M x K (A stored K x M when ta),
+ * op(B) K x N (B stored N x K when tb); C is read and overwritten. */
+int vsvb_debug_gemm(int32_t M, int32_t N, int32_t K, int32_t ta, int32_t tb, double alpha, const double *A,
+                    const double *B, double beta, double *C);
 /* Library version string. */
 const char *vsvb_version(void);
 

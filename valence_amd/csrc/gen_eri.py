@@ -32,6 +32,7 @@ from __future__ import annotations
 
 import json
 import os
+import re
 import sys
 from functools import lru_cache
 from typing import Dict, List, Tuple
@@ -275,6 +276,24 @@ def gen_hrr(la, lb, lc, ld, accname, em: Emitter):
     return out_map
 
 
+# Primitive-loop options per class (both off by default): KET_UNROLL[class] = 2
+# emits two ket primitives per iteration as interleaved independent chains;
+# BOYS_BRANCH_FREE selects boys_eval_bf, which keeps the loop body straight-line.
+KET_UNROLL: Dict[Tuple[int, int, int, int], int] = {}
+BOYS_BRANCH_FREE = False
+
+
+def _suffix_block(lines, suf):
+    """Rename every local declared in a block of generated lines (and K)."""
+    names = set()
+    for ln in lines:
+        if re.match(r"\s*(const double|const PrimPair|double)\s", ln):
+            names.update(re.findall(r"(?:const double|const PrimPair|double|,)\s+(\w+)\s*(?:=(?!=)|\[)", ln))
+    names.add("K")
+    pat = re.compile(r"(?<![.\w])(" + "|".join(sorted(names, key=len, reverse=True)) + r")\b")
+    return [pat.sub(lambda m: m.group(1) + suf, ln) for ln in lines]
+
+
 def gen_class(la, lb, lc, ld, uniform_bra=False):
     """One class's contracted-quartet routine.  By default the ket primitive pairs
     are wave-uniform (scalar loads) and the bra's are per lane; uniform_bra swaps
@@ -299,32 +318,51 @@ def gen_class(la, lb, lc, ld, uniform_bra=False):
     em.emit("      const PrimPair B = " + ("load_uniform(bu + ib);" if uniform_bra else "bp[ib];"))
     if la + lb > 0:
         em.emit("      const double PAx = B.PAx, PAy = B.PAy, PAz = B.PAz, oo2p = B.oo2p;")
-    em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
-    em.emit("        const PrimPair K = " + ("kp[ik];" if uniform_bra else "load_uniform(kc + ik);"))
-    em.emit("        const double s = B.p + K.p;")
-    em.emit("        const double r = rsqrt_fp64(s), rs = r * r;")
-    em.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")
-    em.emit("        const double rho = B.p * K.p * rs;")
-    em.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    em.emit("        const double pref = B.K * K.K * r;")
-    em.emit(f"        double Fb[{L + 1}];")
-    em.emit(f"        boys_eval<{L}>(T, btab, Fb);")
+    blk = Emitter()  # one primitive quartet; K is the ket primitive pair
+    blk.emit("        const PrimPair K = " + ("kp[IK];" if uniform_bra else "load_uniform(kc + IK);"))
+    blk.emit("        const double s = B.p + K.p;")
+    blk.emit("        const double r = rsqrt_fp64(s), rs = r * r;")
+    blk.emit("        const double PQx = B.Px - K.Px, PQy = B.Py - K.Py, PQz = B.Pz - K.Pz;")
+    blk.emit("        const double rho = B.p * K.p * rs;")
+    blk.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    blk.emit("        const double pref = B.K * K.K * r;")
+    blk.emit(f"        double Fb[{L + 1}];")
+    blk.emit(f"        " + ("boys_eval_bf" if BOYS_BRANCH_FREE else "boys_eval") + f"<{L}>(T, btab, Fb);")
     for m in range(L + 1):
-        em.emit(f"        const double F{m} = pref * Fb[{m}];")
+        blk.emit(f"        const double F{m} = pref * Fb[{m}];")
     if la + lb > 0:
-        em.emit("        const double WPx = -K.p * rs * PQx, WPy = -K.p * rs * PQy, WPz = -K.p * rs * PQz;")
-        em.emit("        const double rop = K.p * rs;")
+        blk.emit("        const double WPx = -K.p * rs * PQx, WPy = -K.p * rs * PQy, WPz = -K.p * rs * PQz;")
+        blk.emit("        const double rop = K.p * rs;")
     if lc + ld > 0:
-        em.emit("        const double QCx = K.PAx, QCy = K.PAy, QCz = K.PAz, oo2q = K.oo2p;")
-        em.emit("        const double WQx = B.p * rs * PQx, WQy = B.p * rs * PQy, WQz = B.p * rs * PQz;")
-        em.emit("        const double roq = B.p * rs;")
+        blk.emit("        const double QCx = K.PAx, QCy = K.PAy, QCz = K.PAz, oo2q = K.oo2p;")
+        blk.emit("        const double WQx = B.p * rs * PQx, WQy = B.p * rs * PQy, WQz = B.p * rs * PQz;")
+        blk.emit("        const double roq = B.p * rs;")
         if la + lb > 0:
-            em.emit("        const double oo2s = 0.5 * rs;")
+            blk.emit("        const double oo2s = 0.5 * rs;")
     for ln in body.lines:
-        em.emit("    " + ln)
+        blk.emit("    " + ln)
     for t in targets:
-        em.emit(f"        {acc[t]} += {accmap[t]};")
-    em.emit("      }")
+        blk.emit(f"        {acc[t]} += {accmap[t]};")
+    unroll = KET_UNROLL.get((la, lb, lc, ld), 1)
+    if unroll == 2:
+        b0 = _suffix_block(blk.lines, "_u0")
+        b1 = _suffix_block(blk.lines, "_u1")
+        em.emit("      int ik = 0;")
+        em.emit("      for (; ik + 1 < nkp; ik += 2) {")
+        for x, y in zip(b0, b1):  # interleaved: two independent chains
+            em.emit(x.replace("IK", "ik"))
+            em.emit(y.replace("IK", "ik + 1"))
+        em.emit("      }")
+        em.emit("      if (ik < nkp) {")
+        for x in b0:
+            em.emit(x.replace("IK", "ik"))
+        em.emit("      }")
+    else:
+        em.emit("      for (int ik = 0; ik < nkp; ++ik) {")
+        for x in blk.lines:
+            em.emit(x.replace("IK", "ik"))
+        em.emit("      }")
+    em.flops += blk.flops
     em.emit("    }")
     hr = Emitter()
     gen_hrr(la, lb, lc, ld, acc, hr)

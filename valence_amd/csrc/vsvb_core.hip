@@ -429,42 +429,75 @@ __global__ void k_shell_screen(const double* __restrict__ Ct, const double* __re
 // ---------------------------------------------------------------------------
 // device: small dense algebra (row-major)
 // ---------------------------------------------------------------------------
-// C[M x N] = alpha * op(A) * op(B) + beta * C;  op(A) is M x K.
+// C[M x N] = alpha * op(A) * op(B) + beta * C;  op(A) is M x K; FP64 matrix cores.
+// A workgroup computes a 64 x 64 tile of C, each of its four wavefronts a 32 x 32
+// block as 2 x 2 tiles of V_MFMA_F64_16X16X4F64; A and B pass through LDS in
+// steps of 16 along K (zero-filled past the edges, transposes applied there).
+// Lane layout of the instruction (probed on the hardware): A[i][k] is held by
+// lane i + 16k, B[k][j] by lane j + 16k, and D[i][j] by lane j + 16 (i mod 4)
+// in its accumulator register i / 4.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
                                               int ta, const double* __restrict__ B, int ldb, int tb, double beta,
                                               double* __restrict__ C, int ldc) {
-  __shared__ double As[16][17], Bs[16][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
-  double acc = 0.0;
+  __shared__ double As[64][17], Bs[16][65];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < K; k0 += 16) {
-    {
-      const int r = blockIdx.y * 16 + ty, k = k0 + tx;
+#pragma unroll
+    for (int e = t; e < 64 * 16; e += 256) {
+      const int r = e >> 4, k = e & 15;  // A tile element (row r, column k)
+      const int gr = m0 + r, gk = k0 + k;
       double v = 0.0;
-      if (r < M && k < K) v = ta ? A[(size_t)k * lda + r] : A[(size_t)r * lda + k];
-      As[ty][tx] = v;
+      if (gr < M && gk < K) v = ta ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk];
+      As[r][k] = v;
     }
-    {
-      const int k = k0 + ty, c = blockIdx.x * 16 + tx;
+#pragma unroll
+    for (int e = t; e < 16 * 64; e += 256) {
+      const int k = e >> 6, c = e & 63;  // B tile element (row k, column c)
+      const int gk = k0 + k, gc = n0 + c;
       double v = 0.0;
-      if (k < K && c < N) v = tb ? B[(size_t)c * ldb + k] : B[(size_t)k * ldb + c];
-      Bs[ty][tx] = v;
+      if (gk < K && gc < N) v = tb ? B[(size_t)gc * ldb + gk] : B[(size_t)gk * ldb + gc];
+      Bs[k][c] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    for (int kk = 0; kk < 16; kk += 4) {
+      const double a0 = As[wr + li][kk + lk], a1 = As[wr + 16 + li][kk + lk];
+      const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
     __syncthreads();
   }
-  if (row < M && col < N) {
-    double* c = C + (size_t)row * ldc + col;
-    *c = alpha * acc + (beta == 0.0 ? 0.0 : beta * *c);
-  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wr + 16 * a + lk + 4 * r, col = n0 + wc + 16 * b + li;
+        if (row < M && col < N) {
+          double* c = C + (size_t)row * ldc + col;
+          *c = alpha * acc[a][b][r] + (beta == 0.0 ? 0.0 : beta * *c);
+        }
+      }
 }
 
 hipError_t vsvb::gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
                       const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  dim3 g((N + 15) / 16, (M + 15) / 16);
+  dim3 g((N + 63) / 64, (M + 63) / 64);
   hipLaunchKernelGGL(k_gemm, g, dim3(256), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
   return hipGetLastError();
 }
@@ -1466,8 +1499,9 @@ static int batch_gather(vsvb_batch* b) {
 // (classes with different bra and ket classes), and the cooperative kernel
 // (classes with step tables) with automatic, 1-, 8- or 64-unit claims.  The
 // fastest (best of two launches) becomes the class policy of every group of
-// the bucket.  The launches accumulate into the leader's J/K arrays, which the
-// first run clears again.
+// the bucket (a candidate already 1.5x slower than the best after one launch
+// gets no second).  The launches accumulate into the leader's J/K arrays,
+// which the first run clears again.
 struct Policy {
   bool coop = false;
   int per_cu = 0;
@@ -1544,6 +1578,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
           float ms = 0.0f;
           HIP_OK(hipEventElapsedTime(&ms, e0, e1));
           tmin = std::min(tmin, ms);
+          if (tmin > 1.5 * tbest) break;  // clearly slower: no second launch
         }
         if (k == 0) tdef = tmin;
         if (tmin < tbest) {
@@ -1735,6 +1770,23 @@ int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out, dout.p, (size_t)n * (L + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  return VSVB_OK;
+}
+
+int vsvb_debug_gemm(int32_t M, int32_t N, int32_t K, int32_t ta, int32_t tb, double alpha, const double* A,
+                    const double* B, double beta, double* C) {
+  if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return fail(VSVB_ERR_ARG, "invalid GEMM probe arguments");
+  const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
+  DevBuf<double> dA, dB, dC;
+  HIP_OK(dA.alloc(na));
+  HIP_OK(dB.alloc(nb));
+  HIP_OK(dC.alloc(nc));
+  HIP_OK(hipMemcpy(dA.p, A, na * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dB.p, B, nb * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dC.p, C, nc * sizeof(double), hipMemcpyHostToDevice));
+  // row-major storage: op(A) is M x K, so A is stored K x M when transposed
+  HIP_OK(gemm(nullptr, M, N, K, alpha, dA.p, ta ? M : K, ta != 0, dB.p, tb ? K : N, tb != 0, beta, dC.p, N));
+  HIP_OK(hipMemcpy(C, dC.p, nc * sizeof(double), hipMemcpyDeviceToHost));
   return VSVB_OK;
 }
 

@@ -118,6 +118,42 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
   }
 }
 
+// Branch-free form of boys_eval: the table and the asymptotic evaluations are
+// both computed and the one that applies is selected (the same operations on
+// each path, so the same results), which keeps the calling loop body
+// straight-line code that the compiler can interleave across primitives.
+template <int L>
+__device__ __forceinline__ void boys_eval_bf(double T, const double* __restrict__ tab, double* F) {
+  const bool small = T < kBoysTMax;
+  const double Ts = small ? T : 0.0;
+  const int i = __double2int_rn(Ts * kBoysInvDT);
+  const double d = i * kBoysDT - Ts;
+  const double* t = tab + i * kBoysK + L;
+  double f = t[7], g = t[8];
+#pragma unroll
+  for (int k = 6; k >= 0; --k) {
+    const double dk = d * (1.0 / (k + 1));
+    f = __builtin_fma(f, dk, t[k]);
+    if constexpr (L > 0) g = __builtin_fma(g, dk, t[k + 1]);
+  }
+  double Ft[L + 1];
+  Ft[L] = f;
+  if constexpr (L > 0) {
+    const double t2 = 2.0 * Ts;
+    const double e = __builtin_fma(-t2, g, (2 * L + 1) * f);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) Ft[m] = __builtin_fma(t2, Ft[m + 1], e) * (1.0 / (2 * m + 1));
+  }
+  const double r = rsqrt_fp64(small ? 1.0 : T);
+  double Fa = 0.88622692545275801365 * r;  // sqrt(pi)/2
+  const double h = 0.5 * r * r;
+#pragma unroll
+  for (int m = 0; m <= L; ++m) {
+    F[m] = small ? Ft[m] : Fa;
+    Fa = ((2 * m + 1) * h) * Fa;
+  }
+}
+
 __host__ __device__ __forceinline__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
 }  // namespace vsvb

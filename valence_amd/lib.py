@@ -48,7 +48,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
            "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
-           "vsvb_batch_set_lanes"]
+           "vsvb_batch_set_lanes", "vsvb_debug_gemm"]
 
 _lib = None
 
@@ -78,6 +78,8 @@ def load(path: str = LIB_PATH):
     L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
     L.vsvb_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_debug_boys.argtypes = [C.c_int32, _f64p, C.c_int32, _f64p]
+    L.vsvb_debug_gemm.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, _f64p, _f64p,
+                                  C.c_double, _f64p]
     L.vsvb_class_stats.argtypes = [C.c_void_p, _f64p]
     L.vsvb_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_batch_class_stats.argtypes = [C.c_void_p, _f64p]
@@ -106,6 +108,18 @@ def device_boys(L: int, T) -> np.ndarray:
     out = np.zeros((len(T), L + 1))
     _check(load().vsvb_debug_boys(L, _ptr(T, _f64p), len(T), _ptr(out, _f64p)), "vsvb_debug_boys")
     return out
+
+
+def device_gemm(A: np.ndarray, B: np.ndarray, ta: bool = False, tb: bool = False, alpha: float = 1.0,
+                beta: float = 0.0, C0: Optional[np.ndarray] = None) -> np.ndarray:
+    """alpha op(A) op(B) + beta C0 through the library's device GEMM (tests)."""
+    A = np.ascontiguousarray(A, np.float64); B = np.ascontiguousarray(B, np.float64)
+    M, K = (A.shape[1], A.shape[0]) if ta else A.shape
+    N = B.shape[0] if tb else B.shape[1]
+    C = np.zeros((M, N)) if C0 is None else np.array(C0, np.float64, copy=True, order="C")
+    _check(load().vsvb_debug_gemm(M, N, K, int(ta), int(tb), alpha, _ptr(A, _f64p), _ptr(B, _f64p), beta,
+                                  _ptr(C, _f64p)), "vsvb_debug_gemm")
+    return C
 
 
 class _Packed:
