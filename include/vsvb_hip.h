@@ -106,6 +106,28 @@ int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coef
 int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
                             double *grad_device, void *stream);
 
+/* Spin-coupling matrices of a spin-coupled wave function (nspinc couplings),
+ * replacing the ham/ovl accumulation of the reference's spin optimization
+ * (spin_opt, src/valence.F90:850-936; density's spinopt branch, :1549-1569):
+ * ham[I][J] = the energy numerator E1 + E2 over bra determinants of coupling I
+ * and ket determinants of coupling J, without the coupling weights; ovl[I][J]
+ * the same for the overlap (the reference's ovl after its division by nelec).
+ * Both [nspinc][nspinc], symmetric, electronic (no E_nuc), for orbitals
+ * normalized as the reference's normal does: for weights c,
+ * E = c.ham.c / c.ovl.c + E_nuc.  coeff: NULL = the context's weights;
+ * geometry: the context's current coordinates. */
+int vsvb_spin_matrices(vsvb_context *ctx, const double *coeff, double *ham, double *ovl);
+/* The reference's spin_opt step: the lowest root of ham c = E ovl c.  energy =
+ * that root + E_nuc; coeff_sc[nspinc] = its eigenvector, normalized to
+ * c.ovl.c = 1 with its largest component positive.  The context keeps its
+ * weights (the reference overwrites coeff_sc, :931-934): pass them to
+ * vsvb_set_spin_coupling to do the same.  VSVB_ERR_NUMERIC when ovl is not
+ * positive definite (the reference's rsg failure, :913-927). */
+int vsvb_spin_opt(vsvb_context *ctx, const double *coeff, double *energy, double *coeff_sc);
+/* New spin-coupling weights coeff_sc[nspinc] for the later calls on this
+ * context (the reference's coeff_sc, set by spin_opt, :931-934). */
+int vsvb_set_spin_coupling(vsvb_context *ctx, const double *coeff_sc);
+
 /* Batched evaluation of many wave functions of uneven size (displaced
  * geometries, trial orbital rotations, different molecules); no counterpart in
  * the reference, which evaluates one wave function per process call.  Wave

@@ -1773,6 +1773,129 @@ int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
   return VSVB_OK;
 }
 
+// Lowest root of H c = E S c (S symmetric positive definite, m small): Cholesky
+// S = L L^T, cyclic Jacobi rotations on M = L^-1 H L^-T, c = L^-T y.  Returns
+// false when S is not positive definite.
+static bool lowest_root(int m, const std::vector<double>& H, const std::vector<double>& S, double* eig,
+                        double* vec) {
+  std::vector<double> L((size_t)m * m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    double d = S[(size_t)j * m + j];
+    for (int k = 0; k < j; ++k) d -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
+    if (!(d > 0.0)) return false;
+    L[(size_t)j * m + j] = std::sqrt(d);
+    for (int i = j + 1; i < m; ++i) {
+      double v = S[(size_t)i * m + j];
+      for (int k = 0; k < j; ++k) v -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
+      L[(size_t)i * m + j] = v / L[(size_t)j * m + j];
+    }
+  }
+  // W = L^-1 H (forward substitution per column), M = W L^-T (per row)
+  std::vector<double> W((size_t)m * m), M((size_t)m * m);
+  for (int c = 0; c < m; ++c)
+    for (int i = 0; i < m; ++i) {
+      double v = H[(size_t)i * m + c];
+      for (int k = 0; k < i; ++k) v -= L[(size_t)i * m + k] * W[(size_t)k * m + c];
+      W[(size_t)i * m + c] = v / L[(size_t)i * m + i];
+    }
+  for (int r = 0; r < m; ++r)
+    for (int i = 0; i < m; ++i) {
+      double v = W[(size_t)r * m + i];
+      for (int k = 0; k < i; ++k) v -= L[(size_t)i * m + k] * M[(size_t)r * m + k];
+      M[(size_t)r * m + i] = v / L[(size_t)i * m + i];
+    }
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < i; ++j) M[(size_t)i * m + j] = M[(size_t)j * m + i] = 0.5 * (M[(size_t)i * m + j] + M[(size_t)j * m + i]);
+  std::vector<double> V((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        tot += M[(size_t)i * m + j] * M[(size_t)i * m + j];
+        if (i != j) off += M[(size_t)i * m + j] * M[(size_t)i * m + j];
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = M[(size_t)p * m + q];
+        if (apq == 0.0) continue;
+        const double th = 0.5 * (M[(size_t)q * m + q] - M[(size_t)p * m + p]) / apq;
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < m; ++k) {  // M <- J^T M J, V <- V J
+          const double mkp = M[(size_t)k * m + p], mkq = M[(size_t)k * m + q];
+          M[(size_t)k * m + p] = cs * mkp - sn * mkq;
+          M[(size_t)k * m + q] = sn * mkp + cs * mkq;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double mpk = M[(size_t)p * m + k], mqk = M[(size_t)q * m + k];
+          M[(size_t)p * m + k] = cs * mpk - sn * mqk;
+          M[(size_t)q * m + k] = sn * mpk + cs * mqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double vkp = V[(size_t)k * m + p], vkq = V[(size_t)k * m + q];
+          V[(size_t)k * m + p] = cs * vkp - sn * vkq;
+          V[(size_t)k * m + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  int lo = 0;
+  for (int i = 1; i < m; ++i)
+    if (M[(size_t)i * m + i] < M[(size_t)lo * m + lo]) lo = i;
+  *eig = M[(size_t)lo * m + lo];
+  // c = L^-T y (back substitution), y = column lo of V
+  for (int i = m - 1; i >= 0; --i) {
+    double v = V[(size_t)i * m + lo];
+    for (int k = i + 1; k < m; ++k) v -= L[(size_t)k * m + i] * vec[k];
+    vec[i] = v / L[(size_t)i * m + i];
+  }
+  int big = 0;
+  for (int i = 1; i < m; ++i)
+    if (std::fabs(vec[i]) > std::fabs(vec[big])) big = i;
+  if (vec[big] < 0.0)
+    for (int i = 0; i < m; ++i) vec[i] = -vec[i];
+  return true;
+}
+
+static int spin_context(vsvb_context* c, const double* coeff) {
+  if (c->npair == 0) return fail(VSVB_ERR_ARG, "the wave function has no spin-coupled pairs");
+  int rc;
+  HIP_OK(hipSetDevice(c->device));
+  if ((rc = prepare_geometry(c, nullptr))) return rc;
+  return prepare_orbital_basis(c, coeff, nullptr);
+}
+
+int vsvb_spin_matrices(vsvb_context* c, const double* coeff, double* ham, double* ovl) {
+  if (!c || !ham || !ovl) return fail(VSVB_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = spin_context(c, coeff))) return rc;
+  return sc_matrices(c, ham, ovl);
+}
+
+int vsvb_spin_opt(vsvb_context* c, const double* coeff, double* energy, double* coeff_sc) {
+  if (!c || !energy || !coeff_sc) return fail(VSVB_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = spin_context(c, coeff))) return rc;
+  const int m = c->sc.nspinc;
+  std::vector<double> H((size_t)m * m), S((size_t)m * m);
+  if ((rc = sc_matrices(c, H.data(), S.data()))) return rc;
+  double eig = 0.0;
+  if (!lowest_root(m, H, S, &eig, coeff_sc))
+    return fail(VSVB_ERR_NUMERIC, "overlap matrix of the spin couplings is not positive definite");
+  *energy = eig + c->enuc;
+  return VSVB_OK;
+}
+
+int vsvb_set_spin_coupling(vsvb_context* c, const double* coeff_sc) {
+  if (!c || !coeff_sc) return fail(VSVB_ERR_ARG, "null argument");
+  if (c->npair == 0) return fail(VSVB_ERR_ARG, "the wave function has no spin-coupled pairs");
+  for (int i = 0; i < c->sc.nspinc; ++i)
+    if (!std::isfinite(coeff_sc[i])) return fail(VSVB_ERR_ARG, "non-finite spin-coupling weight");
+  HIP_OK(hipSetDevice(c->device));
+  return sc_set_coupling(c, coeff_sc);
+}
+
 int vsvb_debug_gemm(int32_t M, int32_t N, int32_t K, int32_t ta, int32_t tb, double alpha, const double* A,
                     const double* B, double beta, double* C) {
   if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return fail(VSVB_ERR_ARG, "invalid GEMM probe arguments");

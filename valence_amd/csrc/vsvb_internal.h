@@ -70,7 +70,10 @@ struct DevBuf {
 // coeff_sc[isc].  Work arrays are per ordered determinant pair.
 struct SpinCoupled {
   int ndet = 0, na = 0, nb = 0, npq = 0;
+  int nspinc = 0, npair = 0;             // determinant d belongs to spin coupling d >> npair
+  bool want_mats = false;                // the next pass also accumulates the spin-coupling matrices
   long npairs = 0;
+  DevBuf<double> d_ham, d_ovl;           // [nspinc][nspinc] lower triangles (bra coupling >= ket coupling)
   DevBuf<int> d_alist, d_blist;          // [ndet][na], [ndet][nb]
   DevBuf<int> d_pq;                      // [npq][2] unique orbital pairs (r >= s)
   DevBuf<double> d_cdet;                 // [ndet]
@@ -189,5 +192,10 @@ hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int 
 int sc_setup(vsvb_context* c, const vsvb_system* s);
 int sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev);
 int sc_finish(vsvb_context* c, double* energy, double* parts, double* grad_host);
+// Spin-coupling weights coeff_sc[nspinc] into the per-determinant weights.
+int sc_set_coupling(vsvb_context* c, const double* coeff_sc);
+// Spin-coupling matrices (electronic Hamiltonian numerator and overlap,
+// [nspinc][nspinc] symmetric) at the context's geometry and orbitals.
+int sc_matrices(vsvb_context* c, double* ham, double* ovl);
 
 }  // namespace vsvb

@@ -271,12 +271,16 @@ __device__ __forceinline__ double block_sum64(double v) {
 // (JY) and exchange (KY) images:
 //   f = c_S c_T [ sum_s h.C1_s det_s' + 1/2 sum_s T_s det_s' + C1_a . J(C1_b) ],  omega = c_S c_T det_a det_b
 // with C1_s, T_s, det_s the split cofactors above (one wavefront per pair).
+// With ham != nullptr the unweighted f / (c_S c_T) and omega / (c_S c_T) are
+// also added into the spin-coupling matrices at (I, J) = (S >> npair, T >> npair)
+// for I >= J (bra coupling I, ket coupling J).
 __global__ void __launch_bounds__(64) k_pair_energy(const double* __restrict__ h, const double* __restrict__ Y,
                                                     const double* __restrict__ JY, const double* __restrict__ KY,
                                                     const double* __restrict__ q4, const double* __restrict__ cdet,
                                                     int ndet, int n2, long p0, double* __restrict__ phi,
                                                     double* __restrict__ num, double* __restrict__ wout,
-                                                    double* __restrict__ e1w) {
+                                                    double* __restrict__ e1w, int npair, int nspinc,
+                                                    double* __restrict__ ham, double* __restrict__ ovl) {
   const long t = p0 + blockIdx.x;
   const size_t o = (size_t)t * 6 * n2;
   // sums: hY[6], B[2][4] = (rr, 1r, 2r, 12), X[3][3] = Y_a(alpha) . JY_b(beta)
@@ -329,12 +333,20 @@ __global__ void __launch_bounds__(64) k_pair_energy(const double* __restrict__ h
     for (int aa = 0; aa < 3; ++aa)
       for (int bb = 0; bb < 3; ++bb) cross += cf[0][aa] * cf[1][bb] * acc[14 + 3 * aa + bb];
     const double one = e1[0] * det[1] + e1[1] * det[0];
-    const double f = cst * (one + 0.5 * (tt[0] * det[1] + tt[1] * det[0]) + cross);
-    const double om = cst * det[0] * det[1];
+    const double fu = one + 0.5 * (tt[0] * det[1] + tt[1] * det[0]) + cross, omu = det[0] * det[1];
+    const double f = cst * fu;
+    const double om = cst * omu;
     num[t] = f;
     wout[t] = om;
     e1w[t] = cst * one;
     phi[t] = om != 0.0 ? f / om : 0.0;
+    if (ham) {
+      const int I = S >> npair, J = T >> npair;
+      if (I >= J) {
+        unsafeAtomicAdd(&ham[I * nspinc + J], fu);
+        unsafeAtomicAdd(&ovl[I * nspinc + J], omu);
+      }
+    }
   }
 }
 
@@ -559,6 +571,8 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
   sc.nb = s->npair + s->ndocc;
   if (sc.na > kMaxSpinDim) return fail(VSVB_ERR_UNSUPPORTED, "alpha block larger than 48 orbitals");
   sc.ndet = s->nspinc << np;
+  sc.nspinc = s->nspinc;
+  sc.npair = np;
   sc.npairs = (long)sc.ndet * sc.ndet;
   const int ns = c->nsp;
   sc.npq = ns * (ns + 1) / 2;
@@ -632,6 +646,8 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
   HIP_OK(sc.d_JY.alloc(6 * P * n2));
   HIP_OK(sc.d_KY.alloc(6 * P * n2));
   HIP_OK(sc.d_q4.alloc(8 * P));
+  HIP_OK(sc.d_ham.alloc((size_t)sc.nspinc * sc.nspinc));
+  HIP_OK(sc.d_ovl.alloc((size_t)sc.nspinc * sc.nspinc));
   HIP_OK(hipMemcpy(sc.d_alist.p, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(sc.d_blist.p, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(sc.d_cdet.p, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -672,13 +688,18 @@ int vsvb::sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev) {
     hipLaunchKernelGGL(k_det_pairs, dim3(nb), dim3(64), lds_det, st, da, p0);
     HIP_OK(hipGetLastError());
   }
+  if (sc.want_mats) {
+    HIP_OK(launch_fill(st, sc.d_ham.p, (size_t)sc.nspinc * sc.nspinc, 0.0));
+    HIP_OK(launch_fill(st, sc.d_ovl.p, (size_t)sc.nspinc * sc.nspinc, 0.0));
+  }
   // Coulomb and exchange images of the 6 matrices of every pair: [6P x n^2] x [n^2 x n^2]
   HIP_OK(gemm_rows(st, 6 * P, n2, n2, 1.0, sc.d_Y.p, n2, sc.d_eri.p, n2, false, 0.0, sc.d_JY.p, n2));
   HIP_OK(gemm_rows(st, 6 * P, n2, n2, 1.0, sc.d_Y.p, n2, sc.d_erik.p, n2, false, 0.0, sc.d_KY.p, n2));
   for (long p0 = 0; p0 < P; p0 += chunk) {
     const unsigned nb = (unsigned)std::min(chunk, P - p0);
     hipLaunchKernelGGL(k_pair_energy, dim3(nb), dim3(64), 0, st, c->d_Horb.p, sc.d_Y.p, sc.d_JY.p, sc.d_KY.p,
-                       sc.d_q4.p, sc.d_cdet.p, sc.ndet, n2, p0, sc.d_phi.p, sc.d_num.p, sc.d_w.p, sc.d_e1w.p);
+                       sc.d_q4.p, sc.d_cdet.p, sc.ndet, n2, p0, sc.d_phi.p, sc.d_num.p, sc.d_w.p, sc.d_e1w.p,
+                       sc.npair, sc.nspinc, sc.want_mats ? sc.d_ham.p : nullptr, sc.d_ovl.p);
     HIP_OK(hipGetLastError());
   }
   HIP_OK(launch_reduce(st, sc.d_num.p, P, 1, c->d_scal.p + 0));
@@ -730,6 +751,44 @@ int vsvb::sc_enqueue(vsvb_context* c, bool want_grad, double* grad_dev) {
     HIP_OK(gemm(st, ns, nbf, nbf, 1.0, c->d_X.p, nbf, false, M, nbf, false, 1.0, c->d_gradC.p, nbf));
   }
   return launch_gather_grad(c, c->d_gradC.p, grad_dev);
+}
+
+int vsvb::sc_set_coupling(vsvb_context* c, const double* coeff_sc) {
+  SpinCoupled& sc = c->sc;
+  std::vector<double> cd(sc.ndet);
+  for (int d = 0; d < sc.ndet; ++d) cd[d] = coeff_sc[d >> sc.npair];
+  HIP_OK(hipStreamSynchronize(c->stream));
+  HIP_OK(hipMemcpy(sc.d_cdet.p, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice));
+  return VSVB_OK;
+}
+
+int vsvb::sc_matrices(vsvb_context* c, double* ham, double* ovl) {
+  SpinCoupled& sc = c->sc;
+  int rc;
+  sc.want_mats = true;
+  rc = sc_enqueue(c, false, nullptr);
+  sc.want_mats = false;
+  if (rc) return rc;
+  const int m = sc.nspinc, ns = c->nsp;
+  std::vector<double> h((size_t)m * m), o((size_t)m * m), sorb((size_t)ns * ns);
+  HIP_OK(hipMemcpyAsync(h.data(), sc.d_ham.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(o.data(), sc.d_ovl.p, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(sorb.data(), c->d_Sorb.p, sorb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = sc_finish(c, nullptr, nullptr, nullptr))) return rc;  // synchronizes and checks the status words
+  // The reference normalizes every orbital in place (normal, src/valence.F90:2157-2182); the
+  // energy does not depend on it, these matrices do: a singly occupied orbital enters every
+  // determinant pair twice (bra and ket), a doubly occupied one four times.
+  const int nnd = 2 * sc.npair + c->nunpd;
+  double logf = 0.0;
+  for (int p = 0; p < ns; ++p) logf -= (p < nnd ? 1.0 : 2.0) * std::log(sorb[(size_t)p * ns + p]);
+  const double f = std::exp(logf);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      const int a = std::max(i, j), b = std::min(i, j);
+      ham[(size_t)i * m + j] = f * h[(size_t)a * m + b];
+      ovl[(size_t)i * m + j] = f * o[(size_t)a * m + b];
+    }
+  return VSVB_OK;
 }
 
 int vsvb::sc_finish(vsvb_context* c, double* energy, double* parts, double* grad_host) {

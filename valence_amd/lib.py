@@ -48,7 +48,8 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
            "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
-           "vsvb_batch_set_lanes", "vsvb_debug_gemm"]
+           "vsvb_batch_set_lanes", "vsvb_debug_gemm",
+           "vsvb_spin_matrices", "vsvb_spin_opt", "vsvb_set_spin_coupling"]
 
 _lib = None
 
@@ -78,6 +79,9 @@ def load(path: str = LIB_PATH):
     L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
     L.vsvb_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_debug_boys.argtypes = [C.c_int32, _f64p, C.c_int32, _f64p]
+    L.vsvb_spin_matrices.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+    L.vsvb_spin_opt.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_double), _f64p]
+    L.vsvb_set_spin_coupling.argtypes = [C.c_void_p, _f64p]
     L.vsvb_debug_gemm.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, _f64p, _f64p,
                                   C.c_double, _f64p]
     L.vsvb_class_stats.argtypes = [C.c_void_p, _f64p]
@@ -173,6 +177,27 @@ class Context:
         _check(load().vsvb_energy(self._h, _ptr(xy, _f64p), _ptr(cf, _f64p), C.byref(e), _ptr(p, _f64p)),
                "vsvb_energy")
         return (e.value, p) if parts else e.value
+
+    def spin_matrices(self, coeff: Optional[np.ndarray] = None):
+        """(ham, ovl), [nspinc, nspinc] each: E = c.ham.c / c.ovl.c + E_nuc (vsvb_spin_matrices)."""
+        m = int(self.sys.nspinc)
+        h = np.zeros((m, m)); o = np.zeros((m, m))
+        cf = None if coeff is None else np.ascontiguousarray(coeff, np.float64)
+        _check(load().vsvb_spin_matrices(self._h, _ptr(cf, _f64p), _ptr(h, _f64p), _ptr(o, _f64p)),
+               "vsvb_spin_matrices")
+        return h, o
+
+    def spin_opt(self, coeff: Optional[np.ndarray] = None):
+        """(energy, coeff_sc) of the lowest spin-coupling root (vsvb_spin_opt)."""
+        m = int(self.sys.nspinc)
+        c = np.zeros(m); e = C.c_double()
+        cf = None if coeff is None else np.ascontiguousarray(coeff, np.float64)
+        _check(load().vsvb_spin_opt(self._h, _ptr(cf, _f64p), C.byref(e), _ptr(c, _f64p)), "vsvb_spin_opt")
+        return e.value, c
+
+    def set_spin_coupling(self, coeff_sc: np.ndarray):
+        c = np.ascontiguousarray(coeff_sc, np.float64)
+        _check(load().vsvb_set_spin_coupling(self._h, _ptr(c, _f64p)), "vsvb_set_spin_coupling")
 
     def energy_grad(self, coeff: Optional[np.ndarray] = None, coords: Optional[np.ndarray] = None):
         cf = None if coeff is None else np.ascontiguousarray(coeff, np.float64)
