@@ -106,6 +106,14 @@ int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coef
 int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
                             double *grad_device, void *stream);
 
+/* The NITROGEN potential-surface plug-in (src/valence_api_nitrogen.F90):
+ * calcsurface (:17-27), the energy at Cartesian coordinates x[natom][3] in
+ * angstrom (converted with angs2bohr's 1.889725987722, src/tools_module.F90:11),
+ * returned in cm^-1 (x 219474.631).  The wave function is the context's (the
+ * reference runs its optimizer inside the call when the input asks for it;
+ * this is the guess energy of the reference's output). */
+int vsvb_surface(vsvb_context *ctx, const double *x_angstrom, double *v_cm1);
+
 /* Spin-coupling matrices of a spin-coupled wave function (nspinc couplings),
  * replacing the ham/ovl accumulation of the reference's spin optimization
  * (spin_opt, src/valence.F90:850-936; density's spinopt branch, :1549-1569):
@@ -155,6 +163,15 @@ int vsvb_batch_set_graph(vsvb_batch *batch, int32_t on);
  * off).  n = 1 serializes the groups, for per-kernel timing without overlap.
  * No counterpart in the reference. */
 int vsvb_batch_set_lanes(vsvb_batch *batch, int32_t n);
+/* New coordinates (bohr) for every wave function of the batch, concatenated
+ * [nwfn][natom_i][3], for the next runs (no counterpart in the reference: a
+ * batch of displaced geometries, e.g. a finite-difference stencil).  Wave
+ * functions created on one geometry share one quartet pass and must receive
+ * identical coordinates.  A captured graph is released; the next run is direct. */
+int vsvb_batch_set_coords(vsvb_batch *batch, const double *coords);
+/* calcsurface over the whole batch in one run: x_angstrom concatenated as in
+ * vsvb_batch_set_coords, v_cm1[nwfn] energies in cm^-1. */
+int vsvb_surface_batch(vsvb_batch *batch, const double *x_angstrom, double *v_cm1);
 /* Launch choices of the batch tuner, which runs before the first run, for
  * groups of one wave function (bucket 0) and groups of several sharing one
  * quartet pass (bucket 1): out[2][21][8] = per bucket and class (J/K mode)

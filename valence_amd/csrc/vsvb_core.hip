@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -681,6 +682,46 @@ static int class_occupancy(int bc, int kc, bool jk, bool tr, size_t lds) {
   return 1;
 }
 
+static size_t class_scratch(int bc, int kc, bool jk, bool tr) {
+  switch (bc) {
+    case 0: return scratch_b0(kc, jk, tr);
+    case 1: return scratch_b1(kc, jk, tr);
+    case 2: return scratch_b2(kc, jk, tr);
+    case 3: return scratch_b3(kc, jk, tr);
+    case 4: return scratch_b4(kc, jk, tr);
+    case 5: return scratch_b5(kc, jk, tr);
+  }
+  return 0;
+}
+
+// A straight-line kernel that spills more than this per lane is not launched
+// where the class has cooperative step tables: the runtime reserves scratch
+// for a spilling dispatch on every hardware queue that runs one, and the
+// batch's group streams run such classes on several queues at once.
+constexpr size_t kMaxSpillPerLane = 2048;
+
+// Spill bytes per lane of every straight-line class kernel: [class][energy, J/K, J/K transposed].
+static const size_t* straight_spills() {
+  static size_t tab[kNQuartetClass][3];
+  static std::once_flag once;
+  std::call_once(once, [] {
+    for (int bc = 0, q = 0; bc < kNPairClass; ++bc)
+      for (int kc = 0; kc <= bc; ++kc, ++q) {
+        tab[q][0] = class_scratch(bc, kc, false, false);
+        tab[q][1] = class_scratch(bc, kc, true, false);
+        tab[q][2] = bc != kc ? class_scratch(bc, kc, true, true) : tab[q][1];
+      }
+  });
+  return &tab[0][0];
+}
+
+// The straight-line kernel of class q may run (mode jk, orientation tr): it
+// spills little, or the class has no cooperative kernel to run instead.
+static bool straight_allowed(const vsvb_context* c, int q, bool jk, bool tr) {
+  if (c->coop_idx[q] < 0) return true;
+  return straight_spills()[q * 3 + (jk ? (tr ? 2 : 1) : 0)] <= kMaxSpillPerLane;
+}
+
 static hipError_t launch_schwarz(int pc, Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np,
                                  hipStream_t st) {
   switch (pc) {
@@ -954,9 +995,11 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       const int q = quartet_class_index(bc, kc);
       c->coop_idx[q] = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
       const bool coop = coop_default(c->coop_idx[q]);
-      c->le[q] = coop ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false, false);
-      c->ljk[q] = coop ? coop_launch(c, q, true, 0, 0)
-                       : straight_launch(c, bc, kc, true, 0, true, transposed_default(c, bc, kc));
+      const bool tr = transposed_default(c, bc, kc);
+      const bool coop_e = coop || !straight_allowed(c, q, false, false);
+      const bool coop_jk = coop || !straight_allowed(c, q, true, tr);
+      c->le[q] = coop_e ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false, false);
+      c->ljk[q] = coop_jk ? coop_launch(c, q, true, 0, 0) : straight_launch(c, bc, kc, true, 0, true, tr);
       c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
@@ -1400,6 +1443,7 @@ struct vsvb_batch {
   double tuning[kTuneBuckets][kNQuartetClass][kTuneCols] = {};
   bool graph_failed = false;
   bool graph_off = false;        // runs stay direct (vsvb_batch_set_graph)
+  bool graph_hold = false;       // new coordinates were set: the next run is direct and is not captured
   std::string graph_error;
   hipGraphExec_t exec = nullptr;
   std::vector<size_t> goff;      // offset of each wave function's gradient in d_g
@@ -1540,18 +1584,19 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       d0.coop = L->ljk[q].coop;
       d0.tr = L->ljk[q].tr;
       cand.push_back(d0);
+      const bool st_n = straight_allowed(L, q, true, false), st_t = bc != kc && straight_allowed(L, q, true, true);
       for (int pc : {0, 1, 2, 4})
         for (bool lds : {true, false}) {
           Policy p;
           p.per_cu = pc;
           p.lds_on = lds;
-          cand.push_back(p);
-          if (bc != kc && lds) {
+          if (st_n) cand.push_back(p);
+          if (st_t && lds) {
             p.tr = true;
             cand.push_back(p);
           }
         }
-      if (bc != kc) {
+      if (st_t) {
         Policy p;
         p.tr = true;
         p.lds_on = false;
@@ -2165,7 +2210,66 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
   }
   // capturing runs nothing: the graph is built now, after the first (direct) run,
   // and replayed from the next run on; on failure runs stay direct
-  if (!b->graph_failed && !b->graph_off) (void)batch_capture(b);
+  if (!b->graph_failed && !b->graph_off && !b->graph_hold) (void)batch_capture(b);
+  b->graph_hold = false;
+  return VSVB_OK;
+}
+
+int vsvb_batch_set_coords(vsvb_batch* b, const double* coords) {
+  if (!b || !coords) return fail(VSVB_ERR_ARG, "null argument");
+  const size_t n = b->ctx.size();
+  std::vector<size_t> off(n + 1, 0);
+  for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + 3 * (size_t)b->ctx[i]->natom;
+  for (size_t i = 0; i < off[n]; ++i)
+    if (!std::isfinite(coords[i])) return fail(VSVB_ERR_ARG, "non-finite coordinate");
+  // wave functions that share one quartet pass share one geometry
+  for (auto& g : b->groups)
+    for (size_t k = 1; k < g.members.size(); ++k)
+      if (std::memcmp(coords + off[g.members[k]], coords + off[g.members[0]], 3 * sizeof(double) *
+                      b->ctx[g.members[0]]->natom) != 0)
+        return fail(VSVB_ERR_ARG, "wave functions " + std::to_string(g.members[0]) + " and " +
+                                      std::to_string(g.members[k]) +
+                                      " were created on one geometry and share its quartet pass: give them the same "
+                                      "coordinates");
+  HIP_OK(hipSetDevice(b->ctx[0]->device));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (auto s : b->lanes) HIP_OK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; ++i) b->ctx[i]->coords.assign(coords + off[i], coords + off[i + 1]);
+  // a captured graph holds the old nuclear repulsion and skips the coordinate upload
+  if (b->exec) {
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  b->graph_hold = true;
+  return VSVB_OK;
+}
+
+// calcsurface (src/valence_api_nitrogen.F90:17-27): coordinates in angstrom
+// (tools_module angs2bohr factor), energy in cm^-1.
+constexpr double kAngs2Bohr = 1.889725987722;
+constexpr double kHartree2Wavenumber = 219474.631;
+
+int vsvb_surface(vsvb_context* c, const double* x_angstrom, double* v_cm1) {
+  if (!c || !x_angstrom || !v_cm1) return fail(VSVB_ERR_ARG, "null argument");
+  std::vector<double> x(3 * (size_t)c->natom);
+  for (size_t i = 0; i < x.size(); ++i) x[i] = x_angstrom[i] * kAngs2Bohr;
+  double e = 0.0;
+  const int rc = vsvb_energy(c, x.data(), nullptr, &e, nullptr);
+  if (rc) return rc;
+  *v_cm1 = e * kHartree2Wavenumber;
+  return VSVB_OK;
+}
+
+int vsvb_surface_batch(vsvb_batch* b, const double* x_angstrom, double* v_cm1) {
+  if (!b || !x_angstrom || !v_cm1) return fail(VSVB_ERR_ARG, "null argument");
+  size_t tot = 0;
+  for (auto* c : b->ctx) tot += 3 * (size_t)c->natom;
+  std::vector<double> x(tot);
+  for (size_t i = 0; i < tot; ++i) x[i] = x_angstrom[i] * kAngs2Bohr;
+  int rc;
+  if ((rc = vsvb_batch_set_coords(b, x.data()))) return rc;
+  if ((rc = vsvb_batch_run(b, 0, v_cm1, nullptr))) return rc;
+  for (size_t i = 0; i < b->ctx.size(); ++i) v_cm1[i] *= kHartree2Wavenumber;
   return VSVB_OK;
 }
 

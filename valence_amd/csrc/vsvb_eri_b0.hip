@@ -17,6 +17,13 @@ int occupancy_b0(int kc, bool jk, bool tr, size_t lds) {
   }
 }
 
+size_t scratch_b0(int kc, bool jk, bool tr) {
+  switch (kc) {
+    case 0: return scratch_q<0, 0, 0, 0>(jk, tr);
+    default: return 0;
+  }
+}
+
 hipError_t launch_schwarz_b0(Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np, hipStream_t st) {
   if (np <= 0) return hipSuccess;
   hipLaunchKernelGGL((schwarz_kernel<0, 0>), dim3((np + 3) / 4), dim3(256), 0, st, pairs, pp, boys, p0, np);

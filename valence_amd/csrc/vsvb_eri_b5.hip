@@ -27,6 +27,18 @@ int occupancy_b5(int kc, bool jk, bool tr, size_t lds) {
   }
 }
 
+size_t scratch_b5(int kc, bool jk, bool tr) {
+  switch (kc) {
+    case 0: return scratch_q<2, 2, 0, 0>(jk, tr);
+    case 1: return scratch_q<2, 2, 1, 0>(jk, tr);
+    case 2: return scratch_q<2, 2, 1, 1>(jk, tr);
+    case 3: return scratch_q<2, 2, 2, 0>(jk, tr);
+    case 4: return scratch_q<2, 2, 2, 1>(jk, tr);
+    case 5: return scratch_q<2, 2, 2, 2>(jk, tr);
+    default: return 0;
+  }
+}
+
 hipError_t launch_schwarz_b5(Pair* pairs, const PrimPair* pp, const double* boys, int p0, int np, hipStream_t st) {
   if (np <= 0) return hipSuccess;
   hipLaunchKernelGGL((schwarz_kernel<2, 2>), dim3((np + 3) / 4), dim3(256), 0, st, pairs, pp, boys, p0, np);

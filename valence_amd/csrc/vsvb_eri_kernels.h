@@ -654,6 +654,24 @@ static int occupancy_q(bool jk, bool tr, size_t lds_bytes) {
   return n > 0 ? n : 1;
 }
 
+// Private (scratch) bytes per lane of the straight-line kernel of a class in
+// one mode and orientation: the register spills of its generated code.
+template <int LA, int LB, int LC, int LD>
+static size_t scratch_q(bool jk, bool tr) {
+  hipFuncAttributes at{};
+  if constexpr (LA != LC || LB != LD) {
+    if (jk && tr) {
+      (void)hipFuncGetAttributes(&at, reinterpret_cast<const void*>(quartet_jkt_kernel<LA, LB, LC, LD>));
+      return at.localSizeBytes;
+    }
+  }
+  if (jk)
+    (void)hipFuncGetAttributes(&at, reinterpret_cast<const void*>(quartet_kernel<LA, LB, LC, LD, true>));
+  else
+    (void)hipFuncGetAttributes(&at, reinterpret_cast<const void*>(quartet_kernel<LA, LB, LC, LD, false>));
+  return at.localSizeBytes;
+}
+
 template <int LA, int LB, int LC, int LD>
 static hipError_t launch_q(const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds_bytes, hipStream_t st) {
   if (nblocks <= 0 || a.ntiles <= 0) return hipSuccess;
@@ -680,11 +698,17 @@ hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, bool tr, int
 hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
 hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
 int occupancy_b0(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b0(int kc, bool jk, bool tr);
 int occupancy_b1(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b1(int kc, bool jk, bool tr);
 int occupancy_b2(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b2(int kc, bool jk, bool tr);
 int occupancy_b3(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b3(int kc, bool jk, bool tr);
 int occupancy_b4(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b4(int kc, bool jk, bool tr);
 int occupancy_b5(int kc, bool jk, bool tr, size_t lds);
+size_t scratch_b5(int kc, bool jk, bool tr);
 hipError_t launch_schwarz_b0(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b1(Pair*, const PrimPair*, const double*, int, int, hipStream_t);
 hipError_t launch_schwarz_b2(Pair*, const PrimPair*, const double*, int, int, hipStream_t);

@@ -49,7 +49,8 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
            "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
            "vsvb_batch_set_lanes", "vsvb_debug_gemm",
-           "vsvb_spin_matrices", "vsvb_spin_opt", "vsvb_set_spin_coupling"]
+           "vsvb_spin_matrices", "vsvb_spin_opt", "vsvb_set_spin_coupling", "vsvb_surface",
+           "vsvb_batch_set_coords", "vsvb_surface_batch"]
 
 _lib = None
 
@@ -79,6 +80,9 @@ def load(path: str = LIB_PATH):
     L.vsvb_debug_buffer.argtypes = [C.c_void_p, C.c_char_p, _f64p, C.c_int64]
     L.vsvb_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_debug_boys.argtypes = [C.c_int32, _f64p, C.c_int32, _f64p]
+    L.vsvb_surface.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_double)]
+    L.vsvb_batch_set_coords.argtypes = [C.c_void_p, _f64p]
+    L.vsvb_surface_batch.argtypes = [C.c_void_p, _f64p, _f64p]
     L.vsvb_spin_matrices.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
     L.vsvb_spin_opt.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_double), _f64p]
     L.vsvb_set_spin_coupling.argtypes = [C.c_void_p, _f64p]
@@ -178,6 +182,13 @@ class Context:
                "vsvb_energy")
         return (e.value, p) if parts else e.value
 
+    def surface(self, x_angstrom: np.ndarray) -> float:
+        """calcsurface: energy in cm^-1 at Cartesian coordinates in angstrom (vsvb_surface)."""
+        x = np.ascontiguousarray(x_angstrom, np.float64).reshape(-1)
+        v = C.c_double()
+        _check(load().vsvb_surface(self._h, _ptr(x, _f64p), C.byref(v)), "vsvb_surface")
+        return v.value
+
     def spin_matrices(self, coeff: Optional[np.ndarray] = None):
         """(ham, ovl), [nspinc, nspinc] each: E = c.ham.c / c.ovl.c + E_nuc (vsvb_spin_matrices)."""
         m = int(self.sys.nspinc)
@@ -268,6 +279,18 @@ class Batch:
     def set_lanes(self, n: int):
         """Number of streams the groups run on (1 serializes them)."""
         _check(load().vsvb_batch_set_lanes(self._h, int(n)), "vsvb_batch_set_lanes")
+
+    def set_coords(self, coords_bohr):
+        """New coordinates for every wave function (list of [natom, 3] arrays, bohr)."""
+        x = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).reshape(-1) for c in coords_bohr]))
+        _check(load().vsvb_batch_set_coords(self._h, _ptr(x, _f64p)), "vsvb_batch_set_coords")
+
+    def surface(self, x_angstrom) -> np.ndarray:
+        """Energies in cm^-1 of every wave function at new coordinates in angstrom (vsvb_surface_batch)."""
+        x = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).reshape(-1) for c in x_angstrom]))
+        v = np.zeros(len(self.systems))
+        _check(load().vsvb_surface_batch(self._h, _ptr(x, _f64p), _ptr(v, _f64p)), "vsvb_surface_batch")
+        return v
 
     def uses_graph(self) -> bool:
         """True once runs replay the captured HIP graph (captured after the first run)."""
