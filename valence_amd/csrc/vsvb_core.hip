@@ -764,7 +764,8 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
   }
   L.tr = tr && bc != kc;
   const int rows = L.tr ? ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1]) : ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1]);
-  L.units = L.tr ? ((nk + kStripK - 1) / kStripK) * nb : ((nb + kStripB - 1) / kStripB) * nk;
+  L.strips = L.tr ? 1 : (nb + kStripB - 1) / kStripB;
+  L.units = L.tr ? ((nk + kStripK - 1) / kStripK) * nb : L.strips * nk;
   const size_t kb = (size_t)c->nbf * rows * sizeof(double);
   L.lds = (lds_on && kb <= kMaxLdsK) ? kb : 0;
   const int occ = class_occupancy(bc, kc, true, L.tr, L.lds);
@@ -986,7 +987,16 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-  HIP_OK(c->d_tilectr.alloc(kNQuartetClass));
+  // work counters: one per class, or one per bra strip for the J/K normal units
+  c->ctr_total = 0;
+  for (int bc = 0, q = 0; bc < kNPairClass; ++bc)
+    for (int kc = 0; kc <= bc; ++kc, ++q) {
+      const int nb = c->pair_off[bc + 1] - c->pair_off[bc];
+      c->ctr_off[q] = c->ctr_total;
+      c->ctr_len[q] = std::max(1, (nb + kStripB - 1) / kStripB);
+      c->ctr_total += c->ctr_len[q];
+    }
+  HIP_OK(c->d_tilectr.alloc(c->ctr_total));
   HIP_OK(coop_prepare(c->device));
   c->ncu = ncu;
   c->partial_total = 0;
@@ -1191,7 +1201,8 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.jacc = jacc;
   a.kacc = kacc;
   a.counters = c->d_counters.p + 4 * q;
-  a.tile_ctr = c->d_tilectr.p + q;
+  a.tile_ctr = c->d_tilectr.p + c->ctr_off[q];
+  a.nstrip = (jk && !cl.coop && !cl.tr) ? cl.strips : 1;
   return a;
 }
 
@@ -1212,7 +1223,7 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
   if (!jacc) jacc = c->d_J.p;
   if (!kacc) kacc = c->d_K.p;
   HIP_OK(launch_zero(st, c->d_counters.p, 4 * kNQuartetClass * sizeof(unsigned long long)));
-  HIP_OK(launch_zero(st, c->d_tilectr.p, kNQuartetClass * sizeof(int)));
+  HIP_OK(launch_zero(st, c->d_tilectr.p, c->ctr_total * sizeof(int)));
   c->last_jk = jk;
   c->last_nwfn = nwfn;
   c->last_nspin = nspin;
@@ -1615,7 +1626,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
         const QuartetArgs a = class_args(L, bc, kc, cl, true, nwfn, nspin, dens, jacc, kacc, L->d_partial.p);
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
-          HIP_OK(launch_zero(L->stream, L->d_tilectr.p + q, sizeof(int)));
+          HIP_OK(launch_zero(L->stream, L->d_tilectr.p + L->ctr_off[q], L->ctr_len[q] * sizeof(int)));
           HIP_OK(hipEventRecord(e0, L->stream));
           HIP_OK(launch_class(L, bc, kc, cl, a, true));
           HIP_OK(hipEventRecord(e1, L->stream));

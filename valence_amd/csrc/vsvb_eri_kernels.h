@@ -49,7 +49,8 @@ struct QuartetArgs {
   double* jacc;         // J/K mode: Coulomb accumulators [nwfn][nbf][nbf]
   double* kacc;         // J/K mode: exchange accumulators [nwfn][nspin][nbf][nbf]
   unsigned long long* counters;  // [0] quartets evaluated, [1] primitive quartets, [2] skipped
-  int* tile_ctr;        // dynamic tile counter (zero before the launch)
+  int* tile_ctr;        // dynamic tile counter(s), zero before the launch: one, or one per bra strip (J/K normal)
+  int nstrip;           // J/K normal units: bra strips, each with its own counter of ket units
   int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
 };
 
@@ -249,15 +250,45 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
   const int kb_len = lds ? n * NK : 0;
   for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  // Units are (bra strip, ket pair).  Every strip has its own counter of ket
+  // pairs: a workgroup starts on strip blockIdx mod nstrip and moves to the
+  // next strip only when its strip is exhausted, leaving after it has found
+  // every strip exhausted once (counters only grow, so every workgroup ends).
+  // For one density, a lane keeps the Coulomb update of its bra block in
+  // registers while its workgroup stays on one strip.
+  constexpr bool kRegJ = NAB <= 9;
+  const bool racc = kRegJ && a.nwfn == 1;
+  double jreg[kRegJ ? NAB : 1];
+#pragma unroll
+  for (int k = 0; k < (kRegJ ? NAB : 1); ++k) jreg[k] = 0.0;
+  int jst = -1;  // strip whose bra block jreg holds
+  auto flush_j = [&](int s_) {
+    const int ibf = s_ * kStripB + wave * kTileB + lane;
+    if (s_ < 0 || ibf >= a.nbra) return;
+    const Pair Pf = a.pairs[a.bra0 + ibf];
+    const int fa = a.shells[Pf.A].ao, fb = a.shells[Pf.B].ao;
+#pragma unroll
+    for (int ij = 0; ij < (kRegJ ? NAB : 1); ++ij) {
+      if (jreg[ij] != 0.0) unsafeAtomicAdd(&a.jacc[(size_t)(fa + ij / NB) * n + fb + ij % NB], jreg[ij]);
+      jreg[ij] = 0.0;
+    }
+  };
+  int st = (int)(blockIdx.x % (unsigned)a.nstrip), left = a.nstrip;
   for (;;) {
     __syncthreads();  // the previous unit is flushed and s_unit is free
-    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr, 1);
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, 1);
     __syncthreads();
-    const int u = __builtin_amdgcn_readfirstlane(s_unit);
-    if (u >= a.ntiles) break;
-    const int st = u / a.nket;
-    const int ik = __builtin_amdgcn_readfirstlane(u - st * a.nket);
+    const int ik = __builtin_amdgcn_readfirstlane(s_unit);
+    if (ik >= a.nket) {  // this strip is exhausted
+      if (--left == 0) break;
+      st = st + 1 == a.nstrip ? 0 : st + 1;
+      continue;
+    }
     if (a.same && ik > st * kStripB + kStripB - 1) continue;  // the whole strip is above the diagonal
+    if (racc && jst != st) {
+      flush_j(jst);
+      jst = st;
+    }
     const int ib = st * kStripB + wave * kTileB + lane;
     const Pair K = a.pairs[a.ket0 + ik];
     bool act = ib < a.nbra && (!a.same || ik <= ib);
@@ -312,6 +343,12 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
             double t = 0.0;
 #pragma unroll
             for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+            if constexpr (kRegJ) {
+              if (racc) {
+                jreg[ij] += fj * t;
+                continue;
+              }
+            }
             unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
           }
         }
@@ -396,6 +433,7 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
       }
     }
   }
+  if (racc) flush_j(jst);
   if (a.counters) {
     unsigned long long q = nq, pq = nprimq, sk = nskip;
     for (int o = 32; o > 0; o >>= 1) {

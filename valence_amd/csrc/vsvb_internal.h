@@ -100,6 +100,7 @@ struct ClassLaunch {
   int claim = 1;      // cooperative kernel: units per claim
   int per_cu = 0;     // workgroups per CU requested (0 = the kernel's occupancy)
   bool tr = false;    // straight-line J/K: transposed units (one bra pair x 256 ket pairs)
+  int strips = 1;     // straight-line J/K, normal units: bra strips (one work counter each)
 };
 
 }  // namespace vsvb
@@ -142,7 +143,9 @@ struct vsvb_context {
   DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
   DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
   DevBuf<unsigned long long> d_counters;
-  DevBuf<int> d_tilectr;
+  DevBuf<int> d_tilectr;  // work counters: class q owns [ctr_off[q], ctr_off[q] + ctr_len[q])
+  int ctr_off[vsvb::kNQuartetClass] = {0}, ctr_len[vsvb::kNQuartetClass] = {0};
+  int ctr_total = 0;
 
   size_t partial_per_class[vsvb::kNQuartetClass] = {0};
   // launch configuration per quartet class: energy mode and J/K mode
