@@ -764,8 +764,8 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
   }
   L.tr = tr && bc != kc;
   const int rows = L.tr ? ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1]) : ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1]);
-  L.strips = L.tr ? 1 : (nb + kStripB - 1) / kStripB;
-  L.units = L.tr ? ((nk + kStripK - 1) / kStripK) * nb : L.strips * nk;
+  L.strips = L.tr ? (nk + kStripK - 1) / kStripK : (nb + kStripB - 1) / kStripB;
+  L.units = L.strips * (L.tr ? nb : nk);
   const size_t kb = (size_t)c->nbf * rows * sizeof(double);
   L.lds = (lds_on && kb <= kMaxLdsK) ? kb : 0;
   const int occ = class_occupancy(bc, kc, true, L.tr, L.lds);
@@ -987,13 +987,13 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-  // work counters: one per class, or one per bra strip for the J/K normal units
+  // work counters: one per class, or one per strip for the J/K units (bra strips, or ket strips transposed)
   c->ctr_total = 0;
   for (int bc = 0, q = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc, ++q) {
-      const int nb = c->pair_off[bc + 1] - c->pair_off[bc];
+      const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
       c->ctr_off[q] = c->ctr_total;
-      c->ctr_len[q] = std::max(1, (nb + kStripB - 1) / kStripB);
+      c->ctr_len[q] = std::max({1, (nb + kStripB - 1) / kStripB, (nk + kStripK - 1) / kStripK});
       c->ctr_total += c->ctr_len[q];
     }
   HIP_OK(c->d_tilectr.alloc(c->ctr_total));
@@ -1202,7 +1202,7 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.kacc = kacc;
   a.counters = c->d_counters.p + 4 * q;
   a.tile_ctr = c->d_tilectr.p + c->ctr_off[q];
-  a.nstrip = (jk && !cl.coop && !cl.tr) ? cl.strips : 1;
+  a.nstrip = (jk && !cl.coop) ? cl.strips : 1;
   return a;
 }
 

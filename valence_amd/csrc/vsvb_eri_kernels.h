@@ -49,8 +49,8 @@ struct QuartetArgs {
   double* jacc;         // J/K mode: Coulomb accumulators [nwfn][nbf][nbf]
   double* kacc;         // J/K mode: exchange accumulators [nwfn][nspin][nbf][nbf]
   unsigned long long* counters;  // [0] quartets evaluated, [1] primitive quartets, [2] skipped
-  int* tile_ctr;        // dynamic tile counter(s), zero before the launch: one, or one per bra strip (J/K normal)
-  int nstrip;           // J/K normal units: bra strips, each with its own counter of ket units
+  int* tile_ctr;        // dynamic tile counter(s), zero before the launch: one, or one per strip (J/K)
+  int nstrip;           // J/K units: strips (bra, or ket when transposed), each with its own work counter
   int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
 };
 
@@ -477,14 +477,41 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
   const int kb_len = lds ? n * NR : 0;
   for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  // One counter of bra pairs per ket strip, as in the normal units: a
+  // workgroup stays on its ket strip until it is exhausted, and for one
+  // density a lane keeps the Coulomb update of its ket block in registers.
+  constexpr bool kRegJ = NCD <= 9;
+  const bool racc = kRegJ && a.nwfn == 1;
+  double jreg[kRegJ ? NCD : 1];
+#pragma unroll
+  for (int k = 0; k < (kRegJ ? NCD : 1); ++k) jreg[k] = 0.0;
+  int jst = -1;  // ket strip whose ket block jreg holds
+  auto flush_j = [&](int s_) {
+    const int ikf = s_ * kStripK + wave * kTileB + lane;
+    if (s_ < 0 || ikf >= a.nket) return;
+    const Pair Kf = a.pairs[a.ket0 + ikf];
+    const int fc = a.shells[Kf.A].ao, fd = a.shells[Kf.B].ao;
+#pragma unroll
+    for (int kl = 0; kl < (kRegJ ? NCD : 1); ++kl) {
+      if (jreg[kl] != 0.0) unsafeAtomicAdd(&a.jacc[(size_t)(fc + kl / ND) * n + fd + kl % ND], jreg[kl]);
+      jreg[kl] = 0.0;
+    }
+  };
+  int st = (int)(blockIdx.x % (unsigned)a.nstrip), left = a.nstrip;
   for (;;) {
     __syncthreads();  // the previous unit is flushed and s_unit is free
-    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr, 1);
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, 1);
     __syncthreads();
-    const int u = __builtin_amdgcn_readfirstlane(s_unit);
-    if (u >= a.ntiles) break;
-    const int st = u / a.nbra;
-    const int ib = __builtin_amdgcn_readfirstlane(u - st * a.nbra);
+    const int ib = __builtin_amdgcn_readfirstlane(s_unit);
+    if (ib >= a.nbra) {  // this ket strip is exhausted
+      if (--left == 0) break;
+      st = st + 1 == a.nstrip ? 0 : st + 1;
+      continue;
+    }
+    if (racc && jst != st) {
+      flush_j(jst);
+      jst = st;
+    }
     const int ik = st * kStripK + wave * kTileB + lane;
     const Pair P = a.pairs[a.bra0 + ib];
     bool act = ik < a.nket;
@@ -537,6 +564,12 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
             double t = 0.0;
 #pragma unroll
             for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
+            if constexpr (kRegJ) {
+              if (racc) {
+                jreg[kl] += fj * t;
+                continue;
+              }
+            }
             unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], fj * t);
           }
         }
@@ -621,6 +654,7 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
       }
     }
   }
+  if (racc) flush_j(jst);
   if (a.counters) {
     unsigned long long q = nq, pq = nprimq, sk = nskip;
     for (int o = 32; o > 0; o >>= 1) {

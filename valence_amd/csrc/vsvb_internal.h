@@ -100,7 +100,7 @@ struct ClassLaunch {
   int claim = 1;      // cooperative kernel: units per claim
   int per_cu = 0;     // workgroups per CU requested (0 = the kernel's occupancy)
   bool tr = false;    // straight-line J/K: transposed units (one bra pair x 256 ket pairs)
-  int strips = 1;     // straight-line J/K, normal units: bra strips (one work counter each)
+  int strips = 1;     // straight-line J/K: bra strips (normal) or ket strips (transposed), one work counter each
 };
 
 }  // namespace vsvb
