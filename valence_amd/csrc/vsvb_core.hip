@@ -431,75 +431,55 @@ __global__ void k_shell_screen(const double* __restrict__ Ct, const double* __re
 // device: small dense algebra (row-major)
 // ---------------------------------------------------------------------------
 // C[M x N] = alpha * op(A) * op(B) + beta * C;  op(A) is M x K; FP64 matrix cores.
-// A workgroup computes a 64 x 64 tile of C, each of its four wavefronts a 32 x 32
-// block as 2 x 2 tiles of V_MFMA_F64_16X16X4F64; A and B pass through LDS in
-// steps of 16 along K (zero-filled past the edges, transposes applied there).
-// Lane layout of the instruction (probed on the hardware): A[i][k] is held by
-// lane i + 16k, B[k][j] by lane j + 16k, and D[i][j] by lane j + 16 (i mod 4)
-// in its accumulator register i / 4.
+// One wavefront per 16 x 16 tile of C (the products here are small: many
+// tiles keep the CUs busy), V_MFMA_F64_16X16X4F64 over K in steps of 4; A and
+// B pass through LDS in steps of 16 along K (zero-filled past the edges,
+// transposes applied there).  Lane layout of the instruction (probed on the
+// hardware): A[i][k] is held by lane i + 16k, B[k][j] by lane j + 16k, and
+// D[i][j] by lane j + 16 (i mod 4) in its accumulator register i / 4.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
-                                              int ta, const double* __restrict__ B, int ldb, int tb, double beta,
-                                              double* __restrict__ C, int ldc) {
-  __shared__ double As[64][17], Bs[16][65];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+__global__ void __launch_bounds__(64) k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+                                             int ta, const double* __restrict__ B, int ldb, int tb, double beta,
+                                             double* __restrict__ C, int ldc) {
+  __shared__ double As[16][17], Bs[16][17];
+  const int lane = threadIdx.x;
+  const int m0 = blockIdx.y * 16, n0 = blockIdx.x * 16;
   const int li = lane & 15, lk = lane >> 4;
-  v4d acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
-    for (int e = t; e < 64 * 16; e += 256) {
-      const int r = e >> 4, k = e & 15;  // A tile element (row r, column k)
+    for (int e = lane; e < 256; e += 64) {
+      const int r = e >> 4, k = e & 15;  // A tile (row r, column k) and B tile (row r, column k)
+      double va = 0.0, vb = 0.0;
       const int gr = m0 + r, gk = k0 + k;
-      double v = 0.0;
-      if (gr < M && gk < K) v = ta ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk];
-      As[r][k] = v;
-    }
-#pragma unroll
-    for (int e = t; e < 16 * 64; e += 256) {
-      const int k = e >> 6, c = e & 63;  // B tile element (row k, column c)
-      const int gk = k0 + k, gc = n0 + c;
-      double v = 0.0;
-      if (gk < K && gc < N) v = tb ? B[(size_t)gc * ldb + gk] : B[(size_t)gk * ldb + gc];
-      Bs[k][c] = v;
+      if (gr < M && gk < K) va = ta ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk];
+      const int bk = k0 + r, bc = n0 + k;
+      if (bk < K && bc < N) vb = tb ? B[(size_t)bc * ldb + bk] : B[(size_t)bk * ldb + bc];
+      As[r][k] = va;
+      Bs[r][k] = vb;
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
-      const double a0 = As[wr + li][kk + lk], a1 = As[wr + 16 + li][kk + lk];
-      const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-    }
+    for (int kk = 0; kk < 16; kk += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(As[li][kk + lk], Bs[kk + lk][li], acc, 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr + 16 * a + lk + 4 * r, col = n0 + wc + 16 * b + li;
-        if (row < M && col < N) {
-          double* c = C + (size_t)row * ldc + col;
-          *c = alpha * acc[a][b][r] + (beta == 0.0 ? 0.0 : beta * *c);
-        }
-      }
+  for (int r = 0; r < 4; ++r) {
+    const int row = m0 + lk + 4 * r, col = n0 + li;
+    if (row < M && col < N) {
+      double* c = C + (size_t)row * ldc + col;
+      *c = alpha * acc[r] + (beta == 0.0 ? 0.0 : beta * *c);
+    }
+  }
 }
 
 hipError_t vsvb::gemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int lda, bool ta,
                       const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  dim3 g((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL(k_gemm, g, dim3(256), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
+  dim3 g((N + 15) / 16, (M + 15) / 16);
+  hipLaunchKernelGGL(k_gemm, g, dim3(64), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
   return hipGetLastError();
 }
 

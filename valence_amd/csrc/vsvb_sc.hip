@@ -550,7 +550,7 @@ unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(4096, (n + 255) 
 // op(A)[M x K] * op(B)[K x N] with M split into chunks the 2-D grid can hold.
 hipError_t gemm_rows(hipStream_t st, long M, int N, int K, double alpha, const double* A, int lda,
                      const double* B, int ldb, bool tb, double beta, double* C, int ldc) {
-  const long chunk = 65535L * 64;  // rows per launch: the grid's y extent of 64-row tiles
+  const long chunk = 65535L * 16;  // rows per launch: the grid's y extent of 16-row tiles
   for (long r0 = 0; r0 < M; r0 += chunk) {
     const int m = (int)std::min(chunk, M - r0);
     const hipError_t e = gemm(st, m, N, K, alpha, A + (size_t)r0 * lda, lda, false, B, ldb, tb, beta,
