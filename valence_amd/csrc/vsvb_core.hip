@@ -753,6 +753,12 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
   return L;
 }
 
+// Default J/K grid: at most 4 workgroups per CU.  Filling every slot was
+// measured slower on the batch tuner's workloads for the low-AM classes (more
+// concurrent units thrash the caches: (ss|ss) 1.31 -> 1.05 ms, (ds|ss) 0.52 ->
+// 0.30 ms on C8H18); the batch tuner still tries every grid.
+constexpr int kDefaultJkPerCu = 4;
+
 // Default J/K orientation of a straight-line class: transposed when its ket
 // strips keep more lanes busy than the bra strips (few bra pairs), or, at equal
 // lane use, when the bra block is the larger one (fewer per-lane atomics).
@@ -989,7 +995,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       const bool coop_e = coop || !straight_allowed(c, q, false, false);
       const bool coop_jk = coop || !straight_allowed(c, q, true, tr);
       c->le[q] = coop_e ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false, false);
-      c->ljk[q] = coop_jk ? coop_launch(c, q, true, 0, 0) : straight_launch(c, bc, kc, true, 0, true, tr);
+      c->ljk[q] = coop_jk ? coop_launch(c, q, true, 0, 0)
+                          : straight_launch(c, bc, kc, true, kDefaultJkPerCu, true, tr);
       c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
@@ -1562,9 +1569,16 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
   double* jacc = nwfn > 1 ? gs->jacc.p : L->d_J.p;
   double* kacc = nwfn > 1 ? gs->kacc.p : L->d_K.p;
   const int nspin = L->nunpd > 0 ? 2 : 1;
-  hipEvent_t e0, e1;
-  HIP_OK(hipEventCreate(&e0));
-  HIP_OK(hipEventCreate(&e1));
+  struct Events {  // released on every return path
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  HIP_OK(hipEventCreate(&ev.a));
+  HIP_OK(hipEventCreate(&ev.b));
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   best.assign(kNQuartetClass, Policy{});
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
@@ -1574,6 +1588,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       Policy d0;  // the default
       d0.coop = L->ljk[q].coop;
       d0.tr = L->ljk[q].tr;
+      d0.per_cu = L->ljk[q].per_cu;
       cand.push_back(d0);
       const bool st_n = straight_allowed(L, q, true, false), st_t = bc != kc && straight_allowed(L, q, true, true);
       for (int pc : {0, 1, 2, 4})
@@ -1632,8 +1647,6 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       t[6] = tdef;
       t[7] = 1;
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return VSVB_OK;
 }
 
@@ -2107,7 +2120,8 @@ int vsvb_last_counters(vsvb_context* c, double* out4) {
 int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsvb_options* opt, vsvb_batch** b) {
   if (!b || !systems || nwfn < 1) return fail(VSVB_ERR_ARG, "invalid batch arguments");
   *b = nullptr;
-  auto* bt = new vsvb_batch();
+  std::unique_ptr<vsvb_batch> own(new vsvb_batch());  // released on every failure path
+  vsvb_batch* bt = own.get();
   HIP_OK(hipSetDevice(opt ? opt->device : 0));
   HIP_OK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
   for (int i = 0; i < nwfn; ++i) {
@@ -2118,7 +2132,6 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
     if (rc) {
       g_last_error = "wave function " + std::to_string(i) + ": " + g_last_error;
       vsvb_context_destroy(c);
-      delete bt;
       return rc;
     }
     bt->ctx.push_back(c);
@@ -2178,7 +2191,7 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
   HIP_OK(hipHostMalloc((void**)&bt->h_g, bt->goff[nwfn] * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&bt->h_st, 4 * (size_t)nwfn * sizeof(int), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&bt->h_cnt, (size_t)kCnt * nwfn * sizeof(unsigned long long), hipHostMallocDefault));
-  *b = bt;
+  *b = own.release();
   return VSVB_OK;
 }
 
