@@ -64,7 +64,7 @@ typedef struct vsvb_system {
 typedef struct vsvb_options {
     double quartet_tau;  /* AO shell-quartet Schwarz cut: skip if Q_ab*Q_cd < tau (<=0: default 1e-17) */
     int32_t device;      /* HIP device ordinal */
-    int32_t reserved;
+    int32_t ket_chunk;   /* energy-mode ket pairs per work unit: 0 = chosen from the system size, else 1..64 */
 } vsvb_options;
 
 typedef struct vsvb_context vsvb_context;

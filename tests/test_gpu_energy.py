@@ -215,3 +215,25 @@ def test_batch_lanes_match(lib):
             for k in range(len(pick)):
                 assert abs(e[k] - e0[k]) < 1e-10, (n, rep, k, e[k], e0[k])
                 assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])
+
+
+def test_ket_chunk_matches(lib):
+    """Energy units of 64 bra pairs x a chunk of ket pairs (screened kets skipped
+    inside the unit) visit exactly the quartets of one-ket units: the same
+    quartet and skip counts, and the oracle's energy to 1e-9 Eh, for every width."""
+    s = workloads.config1()
+    g = CONFIGS["config1"]
+    ref = None
+    for kch in (1, 3, 16, 64):
+        with lib.Context(s, ket_chunk=kch) as ctx:
+            e = ctx.energy()
+            st = ctx.counters()
+        print(f"ket_chunk={kch}: gpu={e:.12f} quartets={st[0]:.0f} skipped={st[2]:.0f}")
+        assert abs(e - g["energy"]) < 1e-9
+        if ref is None:
+            ref = (e, st[0], st[1], st[2])
+        else:
+            assert abs(e - ref[0]) < 1e-10
+            assert (st[0], st[1], st[2]) == ref[1:]
+    with pytest.raises(lib.VsvbError, match="ket_chunk"):
+        lib.Context(s, ket_chunk=65)

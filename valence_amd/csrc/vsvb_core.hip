@@ -764,22 +764,31 @@ static int class_bra(int q) {
   return kNPairClass - 1;
 }
 
-// Straight-line kernel: energy units are 64-pair bra tiles x ket pairs (one
-// wavefront each, four per workgroup); J/K units are 256-pair bra strips x ket
+// Straight-line kernel: energy units are 64-pair bra tiles x chunks of ket
+// pairs (one wavefront each, four per workgroup); J/K units are 256-pair bra strips x ket
 // pairs (one workgroup each), with the exchange rows of a unit in LDS when
 // lds_on and they fit, or, transposed (tr, bra and ket classes differ), ket
 // strips of 256 pairs x bra pairs.  per_cu = 0: as many workgroups as the
 // kernel's occupancy allows (energy: at most 2048, measured fastest for the
 // low-AM classes).
+constexpr double kEnergyUnitsPerWave = 32.0;
+
 static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on, bool tr) {
   ClassLaunch L;
   const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
   L.per_cu = per_cu;
   if (nb <= 0 || nk <= 0) return L;
   if (!jk) {
-    L.units = ((nb + kTileB - 1) / kTileB) * nk;
     const int occ = class_occupancy(bc, kc, false, false, 0);
     const int cap = per_cu > 0 ? std::min(per_cu, occ) * c->ncu : std::min(2048, occ * c->ncu);
+    const double nbt = (nb + kTileB - 1) / kTileB;
+    // Ket chunks as wide as still leave kEnergyUnitsPerWave units per resident
+    // wavefront: large (and sparse) systems claim up to 64 kets at once and
+    // skip the screened ones inside the unit; small ones keep one ket per unit.
+    int kch = c->ket_chunk > 0 ? c->ket_chunk : 1;
+    while (c->ket_chunk == 0 && kch < 64 && nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap) kch *= 2;
+    L.kchunk = kch;
+    L.units = (int)nbt * ((nk + kch - 1) / kch);
     L.blocks = std::min((L.units + 3) / 4, cap);
     return L;
   }
@@ -836,6 +845,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
   c->device = opt ? opt->device : 0;
   c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
+  c->ket_chunk = opt ? opt->ket_chunk : 0;
+  if (c->ket_chunk < 0 || c->ket_chunk > 64) return fail(VSVB_ERR_ARG, "ket_chunk must be 0 (automatic) or 1..64");
   HIP_OK(hipSetDevice(c->device));
   if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&c->ev0));
@@ -1219,6 +1230,8 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.same = bc == kc;
   a.ntk = cl.claim;
   a.ntiles = cl.units;
+  a.kchunk = cl.kchunk;
+  a.nkc = (a.nket + cl.kchunk - 1) / cl.kchunk;
   a.lds_k = jk && cl.lds > 0;
   a.tau = c->tau;
   a.nbf = c->nbf;

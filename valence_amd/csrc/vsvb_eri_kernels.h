@@ -4,13 +4,12 @@
 // fixed angular-momentum class with the generated Obara-Saika / Head-Gordon-
 // Pople code, then contracts the cartesian block on the fly with the AO
 // densities of every wave function that shares the geometry; no ERI is ever
-// stored.  A 256-thread workgroup takes a tile of 64 bra pairs x 4 ket pairs:
-// each wavefront owns one ket pair (wave-uniform: its pair and primitive data
-// are scalar loads, and the Coulomb update of the ket block is reduced across
-// the wave before one atomic per element) and the 64 bra pairs, one per lane.
-// A grid sized to the occupancy takes tiles from an atomic counter; each
-// wave keeps its own partial sum, so the partials are few (the final sum is
-// over a fixed-size array).  Pairs
+// stored.  In energy mode a wavefront takes a unit of 64 bra pairs (one per
+// lane) and a chunk of ket pairs, visited one at a time (wave-uniform: the
+// ket's pair and primitive data are scalar loads).  A grid sized to the
+// occupancy takes units from an atomic counter; each wave keeps its own
+// partial sum, so the partials are few (the final sum is over a fixed-size
+// array).  Pairs
 // inside a class are sorted by primitive-pair count, then by centre distance,
 // so a wavefront's lanes run primitive loops of the same length and the
 // Schwarz-skipped quartets gather in the same wavefronts.
@@ -38,7 +37,9 @@ struct QuartetArgs {
   int ket0, nket;       // pair index range of the ket class
   int same;             // bra class == ket class: only ket <= bra
   int ntk;              // cooperative kernel: units claimed per wavefront (1..64)
-  int ntiles;           // work units: energy: bra tiles (64 pairs) x ket pairs; J/K: bra strips (256) x ket pairs
+  int ntiles;           // work units: energy: bra tiles (64 pairs) x ket chunks; J/K: bra strips (256) x ket pairs
+  int kchunk;           // energy: ket pairs per unit (1..64)
+  int nkc;              // energy: ket chunks, ceil(nket / kchunk)
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;
   int nwfn;             // wave functions sharing this geometry
@@ -54,15 +55,18 @@ struct QuartetArgs {
   int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
 };
 
-// A work unit is 64 bra pairs x 1 ket pair for one wavefront: the ket pair is
-// wave-uniform, so its pair and primitive data are scalar loads, and the 64 bra
-// pairs take one lane each.
+// Bra pairs of an energy unit (one per lane of a wavefront).
 constexpr int kTileB = 64;
-constexpr int kTileK = 4;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -105,91 +109,115 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
   for (int w = 0; w < kMaxW; ++w) esum[w] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
 
-  // Work units (64 bra pairs x 1 ket pair) are handed to wavefronts one at a
-  // time from an atomic counter, with no barrier, so wavefronts never wait for
-  // each other; in index order units run from the heaviest bra pairs (most
-  // primitive pairs) to the lightest, so the tail is short.
+  // Work units (64 bra pairs x a chunk of kchunk ket pairs) are handed to
+  // wavefronts one at a time from an atomic counter, with no barrier, so
+  // wavefronts never wait for each other; in index order units run from the
+  // heaviest bra pairs (most primitive pairs) to the lightest, so the tail is
+  // short.  With chunks of several kets, one lane per ket tests it against the
+  // largest bra bound of the tile, and the wavefront visits only the kets that
+  // can pass the Schwarz test for some lane: sparse systems, where most
+  // (tile, ket) slots are screened out, then cost one counter claim per chunk
+  // instead of one per ket.
   for (;;) {
     int u = 0;
     if (lane == 0) u = atomicAdd(a.tile_ctr, 1);
     u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
     if (u >= a.ntiles) break;
-    const int tb = u / a.nket;
-    const int ik = __builtin_amdgcn_readfirstlane(u - tb * a.nket);
-    if (a.same && ik > tb * kTileB + kTileB - 1) continue;  // whole wave above the diagonal
+    const int tb = u / a.nkc;
+    const int k0 = __builtin_amdgcn_readfirstlane((u - tb * a.nkc) * a.kchunk);
+    const int klast = tb * kTileB + kTileB - 1;  // same class: last ket any lane of the tile takes
+    if (a.same && k0 > klast) continue;  // whole wave above the diagonal
     const int ib = tb * kTileB + lane;
-    const Pair K = a.pairs[a.ket0 + ik];
-    bool act = ib < a.nbra && (!a.same || ik <= ib);
+    const bool bra_ok = ib < a.nbra;
     Pair P{};
-    if (act) {
-      P = a.pairs[a.bra0 + ib];
-      if (P.Q * K.Q < a.tau) {
+    if (bra_ok) P = a.pairs[a.bra0 + ib];
+    unsigned long long live = 1;
+    if (a.kchunk > 1) {
+      const double qmax = wave_max(bra_ok ? P.Q : 0.0);
+      const int ikl = k0 + lane;
+      bool lv = false;
+      if (lane < a.kchunk && ikl < a.nket && (!a.same || ikl <= klast)) {
+        lv = a.pairs[a.ket0 + ikl].Q * qmax >= a.tau;
+        if (!lv) {  // every bra lane that takes this ket is screened out
+          const int lo = a.same ? max(ikl, tb * kTileB) : tb * kTileB;
+          const int hi = min(a.nbra, tb * kTileB + kTileB);
+          nskip += hi > lo ? hi - lo : 0;
+        }
+      }
+      live = __ballot(lv);
+    }
+    while (live) {
+      const int ik = __builtin_amdgcn_readfirstlane(k0 + __builtin_ctzll(live));
+      live &= live - 1;
+      const Pair K = a.pairs[a.ket0 + ik];
+      bool act = bra_ok && (!a.same || ik <= ib);
+      if (act && P.Q * K.Q < a.tau) {
         act = false;
         ++nskip;
       }
-    }
-    double v[NAB * NCD];
-    if (act) {
-      eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                K.ABz, a.boys, v);
-      ++nq;
-      nprimq += (unsigned long long)P.npp * K.npp;
-    } else {
+      double v[NAB * NCD];
+      if (act) {
+        eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                  K.ABz, a.boys, v);
+        ++nq;
+        nprimq += (unsigned long long)P.npp * K.npp;
+      } else {
 #pragma unroll
-      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
-    }
-    if (!__any(act)) continue;  // wave-uniform: nothing to contract
-    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
-                           : 0.0;
-    // inactive lanes read valid blocks and contribute zero
-    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
-    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
-    for (int w = 0; w < a.nwfn; ++w) {
-      const double* D0 = a.dens + (size_t)w * (a.nspin > 0 ? a.nspin : 1) * nn;
-      double rab[NAB], rcd[NCD];
-#pragma unroll
-      for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-          double t = D0[(size_t)(ma + i) * n + mb + j];
-          if (a.nspin == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
-          rab[i * NB + j] = cfac * t;
-        }
-#pragma unroll
-      for (int k = 0; k < NC; ++k)
-#pragma unroll
-        for (int l = 0; l < ND; ++l) {
-          double t = D0[(size_t)(mc + k) * n + md + l];
-          if (a.nspin == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
-          rcd[k * ND + l] = cfac * t;
-        }
-      double ej = 0.0;
-#pragma unroll
-      for (int ij = 0; ij < NAB; ++ij) {
-        double t = 0.0;
-#pragma unroll
-        for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-        ej += rab[ij] * t;
+        for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
       }
-      double ek = 0.0;
-      for (int s = 0; s < a.nspin; ++s) {
-        DBlocks<NA, NB, NC, ND> d;
-        d.load(D0 + s * nn, n, ma, mb, mc, md);
+      if (!__any(act)) continue;  // wave-uniform: nothing to contract
+      const double deg =
+          act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0) : 0.0;
+      // inactive lanes read valid blocks and contribute zero
+      const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
+      const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+      for (int w = 0; w < a.nwfn; ++w) {
+        const double* D0 = a.dens + (size_t)w * (a.nspin > 0 ? a.nspin : 1) * nn;
+        double rab[NAB], rcd[NCD];
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
+          for (int j = 0; j < NB; ++j) {
+            double t = D0[(size_t)(ma + i) * n + mb + j];
+            if (a.nspin == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
+            rab[i * NB + j] = cfac * t;
+          }
 #pragma unroll
-            for (int k = 0; k < NC; ++k)
+        for (int k = 0; k < NC; ++k)
 #pragma unroll
-              for (int l = 0; l < ND; ++l)
-                ek += v[((i * NB + j) * NC + k) * ND + l] *
-                      (d.ac[i * NC + k] * d.bd[j * ND + l] + d.ad[i * ND + l] * d.bc[j * NC + k]);
+          for (int l = 0; l < ND; ++l) {
+            double t = D0[(size_t)(mc + k) * n + md + l];
+            if (a.nspin == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
+            rcd[k * ND + l] = cfac * t;
+          }
+        double ej = 0.0;
+#pragma unroll
+        for (int ij = 0; ij < NAB; ++ij) {
+          double t = 0.0;
+#pragma unroll
+          for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+          ej += rab[ij] * t;
+        }
+        double ek = 0.0;
+        for (int s = 0; s < a.nspin; ++s) {
+          DBlocks<NA, NB, NC, ND> d;
+          d.load(D0 + s * nn, n, ma, mb, mc, md);
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+              for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l)
+                  ek += v[((i * NB + j) * NC + k) * ND + l] *
+                        (d.ac[i * NC + k] * d.bd[j * ND + l] + d.ad[i * ND + l] * d.bc[j * NC + k]);
+        }
+        const double e = deg * (0.5 * ej - 0.25 * xfac * ek);
+#pragma unroll
+        for (int ww = 0; ww < kMaxW; ++ww)
+          if (ww == w) esum[ww] += e;
       }
-      const double e = deg * (0.5 * ej - 0.25 * xfac * ek);
-#pragma unroll
-      for (int ww = 0; ww < kMaxW; ++ww)
-        if (ww == w) esum[ww] += e;
     }
   }
   {
