@@ -64,7 +64,9 @@ typedef struct vsvb_system {
 typedef struct vsvb_options {
     double quartet_tau;  /* AO shell-quartet Schwarz cut: skip if Q_ab*Q_cd < tau (<=0: default 1e-17) */
     int32_t device;      /* HIP device ordinal */
-    int32_t ket_chunk;   /* energy-mode ket pairs per work unit: 0 = chosen from the system size, else 1..64 */
+    int32_t pair_chunk;  /* pairs of the wave-uniform side per work-unit claim of the straight-line quartet
+                          * kernels (ket pairs; bra pairs for transposed J/K units): 0 = chosen from the
+                          * system size, else 1..64.  Results do not depend on it. */
 } vsvb_options;
 
 typedef struct vsvb_context vsvb_context;

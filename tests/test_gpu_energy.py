@@ -65,6 +65,14 @@ def test_config0_energy(lib):
     assert abs(e - g["energy"]) < 1e-9
 
 
+def _unique_quartets(s):
+    """T(T+1)/2 over the T = nsh(nsh+1)/2 shell pairs: each unique shell quartet
+    is evaluated or skipped by the Schwarz cut exactly once."""
+    nsh = sum(int(s.type_shell_start[t + 1] - s.type_shell_start[t]) for t in s.atom_type)
+    T = nsh * (nsh + 1) // 2
+    return T * (T + 1) // 2
+
+
 def _fd_grad(ctx, c, idx, h=1e-5):
     g = np.zeros(len(idx))
     for n, k in enumerate(idx):
@@ -217,7 +225,7 @@ def test_batch_lanes_match(lib):
                 assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])
 
 
-def test_ket_chunk_matches(lib):
+def test_pair_chunk_matches(lib):
     """Energy units of 64 bra pairs x a chunk of ket pairs (screened kets skipped
     inside the unit) visit exactly the quartets of one-ket units: the same
     quartet and skip counts, and the oracle's energy to 1e-9 Eh, for every width."""
@@ -225,15 +233,36 @@ def test_ket_chunk_matches(lib):
     g = CONFIGS["config1"]
     ref = None
     for kch in (1, 3, 16, 64):
-        with lib.Context(s, ket_chunk=kch) as ctx:
+        with lib.Context(s, pair_chunk=kch) as ctx:
             e = ctx.energy()
             st = ctx.counters()
-        print(f"ket_chunk={kch}: gpu={e:.12f} quartets={st[0]:.0f} skipped={st[2]:.0f}")
+        print(f"pair_chunk={kch}: gpu={e:.12f} quartets={st[0]:.0f} skipped={st[2]:.0f}")
         assert abs(e - g["energy"]) < 1e-9
+        assert int(st[0]) + int(st[2]) == _unique_quartets(s)
         if ref is None:
             ref = (e, st[0], st[1], st[2])
         else:
             assert abs(e - ref[0]) < 1e-10
             assert (st[0], st[1], st[2]) == ref[1:]
-    with pytest.raises(lib.VsvbError, match="ket_chunk"):
-        lib.Context(s, ket_chunk=65)
+    with pytest.raises(lib.VsvbError, match="pair_chunk"):
+        lib.Context(s, pair_chunk=65)
+
+
+def test_pair_chunk_gradient_matches(lib):
+    """J/K units that claim several ket pairs (bra pairs when transposed) and
+    skip the screened ones give the one-pair-claim gradient to 1e-10 relative,
+    with the same quartet and skip counts."""
+    s = workloads.config1()
+    ref = None
+    for kch in (1, 5, 64):
+        with lib.Context(s, pair_chunk=kch) as ctx:
+            e, g = ctx.energy_grad()
+            st = ctx.counters()
+        print(f"pair_chunk={kch}: E={e:.12f} |g|={np.linalg.norm(g):.10e} quartets={st[0]:.0f} skipped={st[2]:.0f}")
+        assert int(st[0]) + int(st[2]) == _unique_quartets(s)
+        if ref is None:
+            ref = (e, g, st[0], st[1], st[2])
+        else:
+            assert abs(e - ref[0]) < 1e-9
+            assert np.linalg.norm(g - ref[1]) <= 1e-10 * np.linalg.norm(ref[1])
+            assert (st[0], st[1], st[2]) == ref[2:]

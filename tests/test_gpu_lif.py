@@ -63,6 +63,14 @@ def _tol(e):
     return 1e-9 * max(1.0, abs(e) / 1e4)
 
 
+def _unique_quartets(s):
+    """Unique shell quartets, T(T+1)/2 over the T = nsh(nsh+1)/2 shell pairs: every
+    one is either evaluated or skipped by the Schwarz cut, exactly once."""
+    nsh = sum(int(s.type_shell_start[t + 1] - s.type_shell_start[t]) for t in s.atom_type)
+    T = nsh * (nsh + 1) // 2
+    return T * (T + 1) // 2
+
+
 def test_lif128_invariances(lib, lif128):
     s = lif128
     rng = np.random.default_rng(11)
@@ -70,6 +78,7 @@ def test_lif128_invariances(lib, lif128):
         e0 = ctx.energy()
         st = ctx.counters()
         print(f"E={e0:.10f} quartets={st[0]:.0f} skipped={st[2]:.0f} eri_ms={st[3]:.2f}", flush=True)
+        assert int(st[0]) + int(st[2]) == _unique_quartets(s)
         # scale 40 random orbitals
         c = s.coeff.copy()
         for p in rng.choice(s.norbs, 40, replace=False):
@@ -101,7 +110,12 @@ def test_lif128_gradient(lib, lif128):
     rng = np.random.default_rng(12)
     with lib.Context(s) as ctx:
         e, g = ctx.energy_grad()
+        stg = ctx.counters()
         e0 = ctx.energy()
+        ste = ctx.counters()
+        print(f"J/K pass: quartets={stg[0]:.0f} skipped={stg[2]:.0f} eri_ms={stg[3]:.2f}", flush=True)
+        assert (stg[0], stg[1], stg[2]) == (ste[0], ste[1], ste[2])
+        assert int(stg[0]) + int(stg[2]) == _unique_quartets(s)
         idx = rng.choice(s.totlen, 4, replace=False)
         fd = np.zeros(len(idx))
         for n, k in enumerate(idx):

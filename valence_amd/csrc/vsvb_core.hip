@@ -765,13 +765,14 @@ static int class_bra(int q) {
 }
 
 // Straight-line kernel: energy units are 64-pair bra tiles x chunks of ket
-// pairs (one wavefront each, four per workgroup); J/K units are 256-pair bra strips x ket
-// pairs (one workgroup each), with the exchange rows of a unit in LDS when
-// lds_on and they fit, or, transposed (tr, bra and ket classes differ), ket
-// strips of 256 pairs x bra pairs.  per_cu = 0: as many workgroups as the
-// kernel's occupancy allows (energy: at most 2048, measured fastest for the
-// low-AM classes).
+// pairs (one wavefront each, four per workgroup); J/K units are 256-pair bra
+// strips x chunks of ket pairs (one workgroup each), with the exchange rows of
+// a unit in LDS when lds_on and they fit, or, transposed (tr, bra and ket
+// classes differ), ket strips of 256 pairs x chunks of bra pairs.  per_cu = 0:
+// as many workgroups as the kernel's occupancy allows (energy: at most 2048,
+// measured fastest for the low-AM classes).
 constexpr double kEnergyUnitsPerWave = 32.0;
+constexpr double kJkUnitsPerBlock = 16.0;
 
 static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on, bool tr) {
   ClassLaunch L;
@@ -785,21 +786,35 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
     // Ket chunks as wide as still leave kEnergyUnitsPerWave units per resident
     // wavefront: large (and sparse) systems claim up to 64 kets at once and
     // skip the screened ones inside the unit; small ones keep one ket per unit.
-    int kch = c->ket_chunk > 0 ? c->ket_chunk : 1;
-    while (c->ket_chunk == 0 && kch < 64 && nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap) kch *= 2;
+    int kch = c->pair_chunk > 0 ? c->pair_chunk : 1;
+    while (c->pair_chunk == 0 && kch < 64 && nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap)
+      kch *= 2;
     L.kchunk = kch;
     L.units = (int)nbt * ((nk + kch - 1) / kch);
+    // several counter segments once claims are chunked (large systems: many
+    // short claims); one counter, heaviest tiles first, otherwise
+    L.seg_tiles = kch > 1 ? kSegTiles : (int)nbt;
+    L.strips = ((int)nbt + L.seg_tiles - 1) / L.seg_tiles;
     L.blocks = std::min((L.units + 3) / 4, cap);
     return L;
   }
   L.tr = tr && bc != kc;
   const int rows = L.tr ? ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1]) : ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1]);
   L.strips = L.tr ? (nk + kStripK - 1) / kStripK : (nb + kStripB - 1) / kStripB;
-  L.units = L.strips * (L.tr ? nb : nk);
+  const int side = L.tr ? nb : nk;  // pairs claimed from each strip's counter
   const size_t kb = (size_t)c->nbf * rows * sizeof(double);
   L.lds = (lds_on && kb <= kMaxLdsK) ? kb : 0;
   const int occ = class_occupancy(bc, kc, true, L.tr, L.lds);
-  L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
+  const int cap = (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu;
+  // Claims of several pairs when that still leaves kJkUnitsPerBlock units per
+  // workgroup (large, sparse systems), one pair per claim otherwise.
+  int kch = c->pair_chunk > 0 ? c->pair_chunk : 1;
+  while (c->pair_chunk == 0 && kch < 64 &&
+         (double)L.strips * ((side + 2 * kch - 1) / (2 * kch)) >= kJkUnitsPerBlock * cap)
+    kch *= 2;
+  L.kchunk = kch;
+  L.units = L.strips * ((side + kch - 1) / kch);
+  L.blocks = std::min(L.units, cap);
   return L;
 }
 
@@ -811,7 +826,9 @@ constexpr int kDefaultJkPerCu = 4;
 
 // Default J/K orientation of a straight-line class: transposed when its ket
 // strips keep more lanes busy than the bra strips (few bra pairs), or, at equal
-// lane use, when the bra block is the larger one (fewer per-lane atomics).
+// lane use, when the bra block is the larger one (fewer per-lane atomics); but
+// not into an orientation whose exchange rows no longer fit in LDS (large
+// basis sets) while the other's do.
 static bool transposed_default(vsvb_context* c, int bc, int kc) {
   if (bc == kc) return false;
   const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
@@ -819,13 +836,20 @@ static bool transposed_default(vsvb_context* c, int bc, int kc) {
   const double ub = (double)nb / (((nb + kStripB - 1) / kStripB) * kStripB);
   const double uk = (double)nk / (((nk + kStripK - 1) / kStripK) * kStripK);
   const int nab = ncart(kPairLs[bc][0]) * ncart(kPairLs[bc][1]), ncd = ncart(kPairLs[kc][0]) * ncart(kPairLs[kc][1]);
-  if (uk > ub + 0.05) return true;
+  // exchange staging: the normal unit holds NC + ND rows of nbf, the transposed one NA + NB
+  const size_t row = (size_t)c->nbf * sizeof(double);
+  const bool fit_n = row * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) <= kMaxLdsK;
+  const bool fit_t = row * (ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1])) <= kMaxLdsK;
+  if (uk > ub + 0.05) return fit_t || !fit_n;
   if (ub > uk + 0.05) return false;
+  if (fit_n != fit_t) return fit_t;
   return nab > ncd;
 }
 
 // Cooperative kernel: one unit per (bra pair, ket pair); claim = 0 picks about
-// units / (4 * workgroups), between 1 and 64.
+// units / (4 * workgroups), between 1 and 64, or, for classes large enough to
+// leave 16 claims of more than 64 units per workgroup (large, mostly screened
+// systems), units / (16 * workgroups) up to 1024.
 static ClassLaunch coop_launch(vsvb_context* c, int q, bool jk, int per_cu, int claim) {
   ClassLaunch L;
   L.coop = true;
@@ -836,7 +860,9 @@ static ClassLaunch coop_launch(vsvb_context* c, int q, bool jk, int per_cu, int 
   L.units = nb * nk;
   const int occ = coop_occupancy(c->coop_idx[q], jk);
   L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
-  L.claim = claim > 0 ? std::min(claim, 64) : std::max(1, std::min(64, L.units / (4 * L.blocks)));
+  const int big = L.units / (16 * L.blocks);
+  L.claim = claim > 0 ? std::min(claim, 64)
+                      : (big > 64 ? std::min(big, 1024) : std::max(1, std::min(64, L.units / (4 * L.blocks))));
   return L;
 }
 
@@ -845,8 +871,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
   c->device = opt ? opt->device : 0;
   c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
-  c->ket_chunk = opt ? opt->ket_chunk : 0;
-  if (c->ket_chunk < 0 || c->ket_chunk > 64) return fail(VSVB_ERR_ARG, "ket_chunk must be 0 (automatic) or 1..64");
+  c->pair_chunk = opt ? opt->pair_chunk : 0;
+  if (c->pair_chunk < 0 || c->pair_chunk > 64) return fail(VSVB_ERR_ARG, "pair_chunk must be 0 (automatic) or 1..64");
   HIP_OK(hipSetDevice(c->device));
   if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&c->ev0));
@@ -1025,7 +1051,9 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-  // work counters: one per class, or one per strip for the J/K units (bra strips, or ket strips transposed)
+  // work counters: one per energy segment of kSegTiles bra tiles, one per J/K
+  // strip (bra strips, or ket strips transposed), one for the cooperative kernel
+  static_assert(kSegTiles * kTileB == kStripB, "an energy segment covers one bra strip");
   c->ctr_total = 0;
   for (int bc = 0, q = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc, ++q) {
@@ -1230,8 +1258,8 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.same = bc == kc;
   a.ntk = cl.claim;
   a.ntiles = cl.units;
-  a.kchunk = cl.kchunk;
-  a.nkc = (a.nket + cl.kchunk - 1) / cl.kchunk;
+  a.kchunk = cl.coop ? 1 : cl.kchunk;
+  a.nkc = (a.nket + a.kchunk - 1) / a.kchunk;
   a.lds_k = jk && cl.lds > 0;
   a.tau = c->tau;
   a.nbf = c->nbf;
@@ -1243,7 +1271,8 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.kacc = kacc;
   a.counters = c->d_counters.p + 4 * q;
   a.tile_ctr = c->d_tilectr.p + c->ctr_off[q];
-  a.nstrip = (jk && !cl.coop) ? cl.strips : 1;
+  a.nstrip = cl.coop ? 1 : cl.strips;
+  a.seg_tiles = cl.seg_tiles;
   return a;
 }
 

@@ -73,172 +73,182 @@ __global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, con
   for (int w = 0; w < kMaxW; ++w) esum[w] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
 
-  // Units are claimed a.ntk (1..64) at a time: each lane applies the Schwarz
-  // test to one of them, and the wavefront then walks the surviving quartets
-  // one by one (large, mostly screened classes claim 64, small ones 1).
+  // Units are claimed a.ntk (1..1024) at a time and tested 64 at a time: each
+  // lane applies the Schwarz test to one of them, and the wavefront then walks
+  // the surviving quartets one by one (large, mostly screened classes claim
+  // many, small ones 1).  A claim of up to 64 is a run of consecutive units; a
+  // larger one is claim number c of nclaims and takes units c, c + nclaims,
+  // c + 2 nclaims, ..., so that the dense corners of the (bra, ket) square
+  // (the heaviest, closest pairs sort first) spread over all claims.
   const int claim = a.ntk;
+  const bool strided = claim > 64;
+  const int nclaims = strided ? (a.ntiles + claim - 1) / claim : 0;
   for (;;) {
-    int u0 = 0;
-    if (lane == 0) u0 = atomicAdd(a.tile_ctr, claim);
-    u0 = __builtin_amdgcn_readfirstlane(__shfl(u0, 0, 64));
-    if (u0 >= a.ntiles) break;
-    const int um = u0 + lane;
-    bool keep = false;
-    if (lane < claim && um < a.ntiles) {
-      const int b0 = um / a.nket, k0 = um - b0 * a.nket;
-      if (!a.same || k0 <= b0) {
-        keep = a.pairs[a.bra0 + b0].Q * a.pairs[a.ket0 + k0].Q >= a.tau;
-        if (!keep) ++nskip;
-      }
-    }
-    unsigned long long live = __ballot(keep);
-    while (live) {
-    const int bit = __builtin_ctzll(live);
-    live &= live - 1;
-    const int u = u0 + bit;
-    const int ib = u / a.nket;
-    const int ik = u - ib * a.nket;
-    const Pair P = a.pairs[a.bra0 + ib];
-    const Pair K = a.pairs[a.ket0 + ik];
-    __syncthreads();  // the previous unit is done with the LDS
-    for (int s = lane; s < d.nT; s += 64) S[s] = 0.0;
-    const UniformPrimPairs bp = as_uniform(a.pp + P.pp0);
-    const UniformPrimPairs kp = as_uniform(a.pp + K.pp0);
-    for (int ipb = 0; ipb < P.npp; ++ipb) {
-      const PrimPair B = load_uniform(bp + ipb);
-      for (int ipk = 0; ipk < K.npp; ++ipk) {
-        const PrimPair Q = load_uniform(kp + ipk);
-        const double r = rsqrt_fp64(B.p + Q.p), rs = r * r;
-        const double PQx = B.Px - Q.Px, PQy = B.Py - Q.Py, PQz = B.Pz - Q.Pz;
-        const double T = B.p * Q.p * rs * (PQx * PQx + PQy * PQy + PQz * PQz);
-        const double pref = B.K * Q.K * r;
-        double F[9];
-        boys_any(L, T, a.boys, F);
-        const double WPx = -Q.p * rs * PQx, WPy = -Q.p * rs * PQy, WPz = -Q.p * rs * PQz;
-        const double WQx = B.p * rs * PQx, WQy = B.p * rs * PQy, WQz = B.p * rs * PQz;
-        const double rop = Q.p * rs, roq = B.p * rs, oo2s = 0.5 * rs;
-        __syncthreads();  // the previous primitive quartet's VRR slots are consumed
-        for (int i = lane; i < d.nbase; i += 64) S[base[2 * i]] = pref * F[base[2 * i + 1]];
-        __syncthreads();
-        for (int l = 0; l < d.nvl; ++l) {
-          for (int i = voff[l] + lane; i < voff[l + 1]; i += 64) {
-            const int4 s0 = *reinterpret_cast<const int4*>(vst + 8 * i);
-            const int4 s1 = *reinterpret_cast<const int4*>(vst + 8 * i + 4);
-            const int code = s1.z;
-            const int kind = code & 3, dir = (code >> 2) & 3, n1 = (code >> 4) & 15, n2 = (code >> 8) & 15;
-            double v;
-            if (kind == 1) {
-              v = pick3(dir, B.PAx, B.PAy, B.PAz) * S[s0.y] + pick3(dir, WPx, WPy, WPz) * S[s0.z];
-              if (n1) v += (double)n1 * B.oo2p * (S[s0.w] - rop * S[s1.x]);
-            } else {
-              v = pick3(dir, Q.PAx, Q.PAy, Q.PAz) * S[s0.y] + pick3(dir, WQx, WQy, WQz) * S[s0.z];
-              if (n1) v += (double)n1 * Q.oo2p * (S[s0.w] - roq * S[s1.x]);
-              if (n2) v += (double)n2 * oo2s * S[s1.y];
-            }
-            S[s0.x] = v;
-          }
-          __syncthreads();
+    int uc = 0;
+    if (lane == 0) uc = atomicAdd(a.tile_ctr, strided ? 1 : claim);
+    uc = __builtin_amdgcn_readfirstlane(__shfl(uc, 0, 64));
+    if (uc >= (strided ? nclaims : a.ntiles)) break;
+    for (int c0 = 0; c0 < claim; c0 += 64) {
+      const long long stride = strided ? nclaims : 1;
+      const long long uml = (long long)uc + (long long)(c0 + lane) * stride;
+      const int um = uml < a.ntiles ? (int)uml : a.ntiles;
+      bool keep = false;
+      if (c0 + lane < claim && um < a.ntiles) {
+        const int b0 = um / a.nket, k0 = um - b0 * a.nket;
+        if (!a.same || k0 <= b0) {
+          keep = a.pairs[a.bra0 + b0].Q * a.pairs[a.ket0 + k0].Q >= a.tau;
+          if (!keep) ++nskip;
         }
-        for (int i = lane; i < d.nT; i += 64) S[targ[2 * i]] += S[targ[2 * i + 1]];
       }
-    }
-    __syncthreads();
-    for (int l = 0; l < d.nhl; ++l) {
-      for (int i = hoff[l] + lane; i < hoff[l + 1]; i += 64) {
-        const int4 h = *reinterpret_cast<const int4*>(hst + 4 * i);
-        const int dir = h.w & 3;
-        const double cf = (h.w & 4) ? pick3(dir, K.ABx, K.ABy, K.ABz) : pick3(dir, P.ABx, P.ABy, P.ABz);
-        S[h.x] = S[h.y] + cf * S[h.z];
+      unsigned long long live = __ballot(keep);
+      while (live) {
+      const int bit = __builtin_ctzll(live);
+      live &= live - 1;
+      const int u = (int)((long long)uc + (long long)(c0 + bit) * stride);
+      const int ib = u / a.nket;
+      const int ik = u - ib * a.nket;
+      const Pair P = a.pairs[a.bra0 + ib];
+      const Pair K = a.pairs[a.ket0 + ik];
+      __syncthreads();  // the previous unit is done with the LDS
+      for (int s = lane; s < d.nT; s += 64) S[s] = 0.0;
+      const UniformPrimPairs bp = as_uniform(a.pp + P.pp0);
+      const UniformPrimPairs kp = as_uniform(a.pp + K.pp0);
+      for (int ipb = 0; ipb < P.npp; ++ipb) {
+        const PrimPair B = load_uniform(bp + ipb);
+        for (int ipk = 0; ipk < K.npp; ++ipk) {
+          const PrimPair Q = load_uniform(kp + ipk);
+          const double r = rsqrt_fp64(B.p + Q.p), rs = r * r;
+          const double PQx = B.Px - Q.Px, PQy = B.Py - Q.Py, PQz = B.Pz - Q.Pz;
+          const double T = B.p * Q.p * rs * (PQx * PQx + PQy * PQy + PQz * PQz);
+          const double pref = B.K * Q.K * r;
+          double F[9];
+          boys_any(L, T, a.boys, F);
+          const double WPx = -Q.p * rs * PQx, WPy = -Q.p * rs * PQy, WPz = -Q.p * rs * PQz;
+          const double WQx = B.p * rs * PQx, WQy = B.p * rs * PQy, WQz = B.p * rs * PQz;
+          const double rop = Q.p * rs, roq = B.p * rs, oo2s = 0.5 * rs;
+          __syncthreads();  // the previous primitive quartet's VRR slots are consumed
+          for (int i = lane; i < d.nbase; i += 64) S[base[2 * i]] = pref * F[base[2 * i + 1]];
+          __syncthreads();
+          for (int l = 0; l < d.nvl; ++l) {
+            for (int i = voff[l] + lane; i < voff[l + 1]; i += 64) {
+              const int4 s0 = *reinterpret_cast<const int4*>(vst + 8 * i);
+              const int4 s1 = *reinterpret_cast<const int4*>(vst + 8 * i + 4);
+              const int code = s1.z;
+              const int kind = code & 3, dir = (code >> 2) & 3, n1 = (code >> 4) & 15, n2 = (code >> 8) & 15;
+              double v;
+              if (kind == 1) {
+                v = pick3(dir, B.PAx, B.PAy, B.PAz) * S[s0.y] + pick3(dir, WPx, WPy, WPz) * S[s0.z];
+                if (n1) v += (double)n1 * B.oo2p * (S[s0.w] - rop * S[s1.x]);
+              } else {
+                v = pick3(dir, Q.PAx, Q.PAy, Q.PAz) * S[s0.y] + pick3(dir, WQx, WQy, WQz) * S[s0.z];
+                if (n1) v += (double)n1 * Q.oo2p * (S[s0.w] - roq * S[s1.x]);
+                if (n2) v += (double)n2 * oo2s * S[s1.y];
+              }
+              S[s0.x] = v;
+            }
+            __syncthreads();
+          }
+          for (int i = lane; i < d.nT; i += 64) S[targ[2 * i]] += S[targ[2 * i + 1]];
+        }
       }
       __syncthreads();
-    }
-    ++nq;
-    nprimq += (unsigned long long)P.npp * K.npp;
-    const double deg = (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0);
-    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
-    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
-    for (int w = 0; w < a.nwfn; ++w) {
-      const double* D0 = a.dens + (size_t)w * dstride * nn;
-      auto rho = [&](int x, int y) {  // cfac * sum_s D_s[x][y]
-        double t = D0[(size_t)x * n + y];
-        if (ns == 2) t += D0[nn + (size_t)x * n + y];
-        return cfac * t;
-      };
-      if constexpr (!kJK) {
-        double e = 0.0;
-        for (int idx = lane; idx < NAB * NCD; idx += 64) {
-          const int ab = idx / NCD, cd = idx - ab * NCD;
-          const int i = ab / NB, j = ab - i * NB, k = cd / ND, l = cd - k * ND;
-          const double v = S[outs[idx]];
-          double ex = 0.0;
+      for (int l = 0; l < d.nhl; ++l) {
+        for (int i = hoff[l] + lane; i < hoff[l + 1]; i += 64) {
+          const int4 h = *reinterpret_cast<const int4*>(hst + 4 * i);
+          const int dir = h.w & 3;
+          const double cf = (h.w & 4) ? pick3(dir, K.ABx, K.ABy, K.ABz) : pick3(dir, P.ABx, P.ABy, P.ABz);
+          S[h.x] = S[h.y] + cf * S[h.z];
+        }
+        __syncthreads();
+      }
+      ++nq;
+      nprimq += (unsigned long long)P.npp * K.npp;
+      const double deg = (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0);
+      const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
+      const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+      for (int w = 0; w < a.nwfn; ++w) {
+        const double* D0 = a.dens + (size_t)w * dstride * nn;
+        auto rho = [&](int x, int y) {  // cfac * sum_s D_s[x][y]
+          double t = D0[(size_t)x * n + y];
+          if (ns == 2) t += D0[nn + (size_t)x * n + y];
+          return cfac * t;
+        };
+        if constexpr (!kJK) {
+          double e = 0.0;
+          for (int idx = lane; idx < NAB * NCD; idx += 64) {
+            const int ab = idx / NCD, cd = idx - ab * NCD;
+            const int i = ab / NB, j = ab - i * NB, k = cd / ND, l = cd - k * ND;
+            const double v = S[outs[idx]];
+            double ex = 0.0;
+            for (int s = 0; s < ns; ++s) {
+              const double* Ds = D0 + s * nn;
+              ex += Ds[(size_t)(ma + i) * n + mc + k] * Ds[(size_t)(mb + j) * n + md + l] +
+                    Ds[(size_t)(ma + i) * n + md + l] * Ds[(size_t)(mb + j) * n + mc + k];
+            }
+            e += v * (0.5 * rho(ma + i, mb + j) * rho(mc + k, md + l) - 0.25 * xfac * ex);
+          }
+#pragma unroll
+          for (int ww = 0; ww < kMaxW; ++ww)
+            if (ww == w) esum[ww] += deg * e;
+        } else {
+          const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+          double* J = a.jacc + (size_t)w * nn;
+          for (int ab = lane; ab < NAB; ab += 64) {
+            const int i = ab / NB, j = ab - i * NB;
+            double t = 0.0;
+            for (int cd = 0; cd < NCD; ++cd) t += S[outs[ab * NCD + cd]] * rho(mc + cd / ND, md + cd % ND);
+            unsafeAtomicAdd(&J[(size_t)(ma + i) * n + mb + j], fj * t);
+          }
+          for (int cd = lane; cd < NCD; cd += 64) {
+            const int k = cd / ND, l = cd - k * ND;
+            double t = 0.0;
+            for (int ab = 0; ab < NAB; ++ab) t += S[outs[ab * NCD + cd]] * rho(ma + ab / NB, mb + ab % NB);
+            unsafeAtomicAdd(&J[(size_t)(mc + k) * n + md + l], fj * t);
+          }
           for (int s = 0; s < ns; ++s) {
             const double* Ds = D0 + s * nn;
-            ex += Ds[(size_t)(ma + i) * n + mc + k] * Ds[(size_t)(mb + j) * n + md + l] +
-                  Ds[(size_t)(ma + i) * n + md + l] * Ds[(size_t)(mb + j) * n + mc + k];
-          }
-          e += v * (0.5 * rho(ma + i, mb + j) * rho(mc + k, md + l) - 0.25 * xfac * ex);
-        }
-#pragma unroll
-        for (int ww = 0; ww < kMaxW; ++ww)
-          if (ww == w) esum[ww] += deg * e;
-      } else {
-        const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
-        double* J = a.jacc + (size_t)w * nn;
-        for (int ab = lane; ab < NAB; ab += 64) {
-          const int i = ab / NB, j = ab - i * NB;
-          double t = 0.0;
-          for (int cd = 0; cd < NCD; ++cd) t += S[outs[ab * NCD + cd]] * rho(mc + cd / ND, md + cd % ND);
-          unsafeAtomicAdd(&J[(size_t)(ma + i) * n + mb + j], fj * t);
-        }
-        for (int cd = lane; cd < NCD; cd += 64) {
-          const int k = cd / ND, l = cd - k * ND;
-          double t = 0.0;
-          for (int ab = 0; ab < NAB; ++ab) t += S[outs[ab * NCD + cd]] * rho(ma + ab / NB, mb + ab % NB);
-          unsafeAtomicAdd(&J[(size_t)(mc + k) * n + md + l], fj * t);
-        }
-        for (int s = 0; s < ns; ++s) {
-          const double* Ds = D0 + s * nn;
-          double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
-          // B[a][c] += v D[b][d]
-          for (int x = lane; x < NA * NC; x += 64) {
-            const int i = x / NC, k = x - i * NC;
-            double t = 0.0;
-            for (int j = 0; j < NB; ++j)
-              for (int l = 0; l < ND; ++l)
-                t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + md + l];
-            unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
-          }
-          // B[b][c] += v D[a][d]
-          for (int x = lane; x < NB * NC; x += 64) {
-            const int j = x / NC, k = x - j * NC;
-            double t = 0.0;
-            for (int i = 0; i < NA; ++i)
-              for (int l = 0; l < ND; ++l)
-                t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + md + l];
-            unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
-          }
-          // B[a][d] += v D[b][c]
-          for (int x = lane; x < NA * ND; x += 64) {
-            const int i = x / ND, l = x - i * ND;
-            double t = 0.0;
-            for (int j = 0; j < NB; ++j)
-              for (int k = 0; k < NC; ++k)
-                t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + mc + k];
-            unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
-          }
-          // B[b][d] += v D[a][c]
-          for (int x = lane; x < NB * ND; x += 64) {
-            const int j = x / ND, l = x - j * ND;
-            double t = 0.0;
-            for (int i = 0; i < NA; ++i)
-              for (int k = 0; k < NC; ++k)
-                t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + mc + k];
-            unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
+            double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+            // B[a][c] += v D[b][d]
+            for (int x = lane; x < NA * NC; x += 64) {
+              const int i = x / NC, k = x - i * NC;
+              double t = 0.0;
+              for (int j = 0; j < NB; ++j)
+                for (int l = 0; l < ND; ++l)
+                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + md + l];
+              unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
+            }
+            // B[b][c] += v D[a][d]
+            for (int x = lane; x < NB * NC; x += 64) {
+              const int j = x / NC, k = x - j * NC;
+              double t = 0.0;
+              for (int i = 0; i < NA; ++i)
+                for (int l = 0; l < ND; ++l)
+                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + md + l];
+              unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
+            }
+            // B[a][d] += v D[b][c]
+            for (int x = lane; x < NA * ND; x += 64) {
+              const int i = x / ND, l = x - i * ND;
+              double t = 0.0;
+              for (int j = 0; j < NB; ++j)
+                for (int k = 0; k < NC; ++k)
+                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + mc + k];
+              unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
+            }
+            // B[b][d] += v D[a][c]
+            for (int x = lane; x < NB * ND; x += 64) {
+              const int j = x / ND, l = x - j * ND;
+              double t = 0.0;
+              for (int i = 0; i < NA; ++i)
+                for (int k = 0; k < NC; ++k)
+                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + mc + k];
+              unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
+            }
           }
         }
       }
-    }
-    }  // surviving units of the claim
+      }  // surviving units of the batch of 64
+    }  // batches of the claim
   }
   if constexpr (!kJK) {
     // one wavefront per workgroup: slot 0 of the block's four wave slots holds the sum

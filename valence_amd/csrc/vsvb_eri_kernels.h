@@ -36,10 +36,11 @@ struct QuartetArgs {
   int bra0, nbra;       // pair index range of the bra class
   int ket0, nket;       // pair index range of the ket class
   int same;             // bra class == ket class: only ket <= bra
-  int ntk;              // cooperative kernel: units claimed per wavefront (1..64)
-  int ntiles;           // work units: energy: bra tiles (64 pairs) x ket chunks; J/K: bra strips (256) x ket pairs
-  int kchunk;           // energy: ket pairs per unit (1..64)
+  int ntk;              // cooperative kernel: units claimed per wavefront (1..1024)
+  int ntiles;           // work units: energy: bra tiles (64 pairs) x ket chunks; J/K: strips x claims
+  int kchunk;           // pairs of the wave-uniform side per claim (1..64): kets; bras in transposed J/K units
   int nkc;              // energy: ket chunks, ceil(nket / kchunk)
+  int seg_tiles;        // energy: bra tiles per counter segment (nstrip segments)
   double tau;           // skip quartet if Q_bra * Q_ket < tau
   int nbf;
   int nwfn;             // wave functions sharing this geometry
@@ -51,12 +52,15 @@ struct QuartetArgs {
   double* kacc;         // J/K mode: exchange accumulators [nwfn][nspin][nbf][nbf]
   unsigned long long* counters;  // [0] quartets evaluated, [1] primitive quartets, [2] skipped
   int* tile_ctr;        // dynamic tile counter(s), zero before the launch: one, or one per strip (J/K)
-  int nstrip;           // J/K units: strips (bra, or ket when transposed), each with its own work counter
+  int nstrip;           // straight-line units: energy segments of kSegTiles bra tiles, J/K strips (bra, or ket
+                        // when transposed), each with its own work counter
   int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
 };
 
-// Bra pairs of an energy unit (one per lane of a wavefront).
+// Bra pairs of an energy unit (one per lane of a wavefront), and bra tiles of
+// an energy counter segment when the class has several.
 constexpr int kTileB = 64;
+constexpr int kSegTiles = 4;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -110,19 +114,32 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
 
   // Work units (64 bra pairs x a chunk of kchunk ket pairs) are handed to
-  // wavefronts one at a time from an atomic counter, with no barrier, so
-  // wavefronts never wait for each other; in index order units run from the
-  // heaviest bra pairs (most primitive pairs) to the lightest, so the tail is
-  // short.  With chunks of several kets, one lane per ket tests it against the
-  // largest bra bound of the tile, and the wavefront visits only the kets that
-  // can pass the Schwarz test for some lane: sparse systems, where most
-  // (tile, ket) slots are screened out, then cost one counter claim per chunk
-  // instead of one per ket.
+  // wavefronts one at a time from atomic counters, with no barrier, so
+  // wavefronts never wait for each other.  The units of every segment of
+  // seg_tiles bra tiles have their own counter: a wavefront starts on segment
+  // (global wave index) mod nstrip and moves to the next one only when its
+  // segment is exhausted, leaving after it has found every segment exhausted
+  // once (counters only grow, so every wavefront ends); spreading the claims
+  // over several counters keeps them from queueing on one address.  Inside a
+  // segment units run from the heaviest bra pairs (most primitive pairs) to
+  // the lightest.  With chunks of several kets, one lane per ket tests it
+  // against the largest bra bound of the tile, and the wavefront visits only
+  // the kets that can pass the Schwarz test for some lane: sparse systems,
+  // where most (tile, ket) slots are screened out, then cost one counter claim
+  // per chunk instead of one per ket.
+  const int nbt = (a.nbra + kTileB - 1) / kTileB;
+  int sg = (int)((blockIdx.x * 4u + (unsigned)wave) % (unsigned)a.nstrip), left = a.nstrip;
   for (;;) {
-    int u = 0;
-    if (lane == 0) u = atomicAdd(a.tile_ctr, 1);
-    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
-    if (u >= a.ntiles) break;
+    int ul = 0;
+    if (lane == 0) ul = atomicAdd(a.tile_ctr + sg, 1);
+    ul = __builtin_amdgcn_readfirstlane(__shfl(ul, 0, 64));
+    const int t0 = sg * a.seg_tiles;
+    if (ul >= min(a.seg_tiles, nbt - t0) * a.nkc) {  // this segment is exhausted
+      if (--left == 0) break;
+      sg = sg + 1 == a.nstrip ? 0 : sg + 1;
+      continue;
+    }
+    const int u = t0 * a.nkc + ul;
     const int tb = u / a.nkc;
     const int k0 = __builtin_amdgcn_readfirstlane((u - tb * a.nkc) * a.kchunk);
     const int klast = tb * kTileB + kTileB - 1;  // same class: last ket any lane of the tile takes
@@ -266,6 +283,7 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
   constexpr int NAB = NA * NB, NCD = NC * ND, NK = NC + ND;
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
+  __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
   const int n = a.nbf;
   const size_t nn = (size_t)n * n;
   const int ns = a.nspin;
@@ -301,162 +319,189 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
       jreg[ij] = 0.0;
     }
   };
+  // A claim takes a chunk of kchunk consecutive ket pairs of the strip; with
+  // more than one, kets that fail the Schwarz test against the strip's largest
+  // bra bound are skipped without a visit (one lane tests each), which spares
+  // sparse systems a claim and an exchange flush per screened-out ket.
   int st = (int)(blockIdx.x % (unsigned)a.nstrip), left = a.nstrip;
   for (;;) {
-    __syncthreads();  // the previous unit is flushed and s_unit is free
-    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, 1);
+    __syncthreads();  // the previous unit is flushed and s_unit, s_qmax are free
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, a.kchunk);
     __syncthreads();
-    const int ik = __builtin_amdgcn_readfirstlane(s_unit);
-    if (ik >= a.nket) {  // this strip is exhausted
+    const int k0 = __builtin_amdgcn_readfirstlane(s_unit);
+    if (k0 >= a.nket) {  // this strip is exhausted
       if (--left == 0) break;
       st = st + 1 == a.nstrip ? 0 : st + 1;
       continue;
     }
-    if (a.same && ik > st * kStripB + kStripB - 1) continue;  // the whole strip is above the diagonal
+    const int klast = st * kStripB + kStripB - 1;  // same class: last ket any lane of the strip takes
+    if (a.same && k0 > klast) continue;            // the whole strip is above the diagonal
     if (racc && jst != st) {
       flush_j(jst);
       jst = st;
     }
-    const int ib = st * kStripB + wave * kTileB + lane;
-    const Pair K = a.pairs[a.ket0 + ik];
-    bool act = ib < a.nbra && (!a.same || ik <= ib);
+    const int tile0 = st * kStripB + wave * kTileB;
+    const int ib = tile0 + lane;
     // every lane holds a pair of the bra class (the last one past the end), so
     // that the density blocks read below stay inside the matrix
     const Pair P = a.pairs[a.bra0 + (ib < a.nbra ? ib : a.nbra - 1)];
-    if (act && P.Q * K.Q < a.tau) {
-      act = false;
-      ++nskip;
-    }
-    double v[NAB * NCD];
-    if (act) {
-      eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                K.ABz, a.boys, v);
-      ++nq;
-      nprimq += (unsigned long long)P.npp * K.npp;
-    } else {
-#pragma unroll
-      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
-    }
-    const bool wact = __any(act);  // wave-uniform
-    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
-                           : 0.0;
-    // inactive lanes read a valid (ket) block and contribute zero
-    const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
-    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
-    const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
-    for (int w = 0; w < a.nwfn; ++w) {
-      const double* D0 = a.dens + (size_t)w * dstride * nn;
-      if (wact) {
-        double rab[NAB], rcd[NCD];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            double t = D0[(size_t)(ma + i) * n + mb + j];
-            if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
-            rab[i * NB + j] = cfac * t;
-          }
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-#pragma unroll
-          for (int l = 0; l < ND; ++l) {
-            double t = D0[(size_t)(mc + k) * n + md + l];
-            if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
-            rcd[k * ND + l] = cfac * t;
-          }
-        double* J = a.jacc + (size_t)w * nn;
-        if (act) {
-#pragma unroll
-          for (int ij = 0; ij < NAB; ++ij) {
-            double t = 0.0;
-#pragma unroll
-            for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-            if constexpr (kRegJ) {
-              if (racc) {
-                jreg[ij] += fj * t;
-                continue;
-              }
-            }
-            unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
-          }
-        }
-#pragma unroll
-        for (int kl = 0; kl < NCD; ++kl) {
-          double t = 0.0;
-#pragma unroll
-          for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
-          t = wave_sum(fj * t);
-          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], t);
+    unsigned long long live = 1;
+    if (a.kchunk > 1) {
+      const double qw = wave_max(ib < a.nbra ? P.Q : 0.0);
+      if (lane == 0) s_qmax[wave] = qw;
+      __syncthreads();
+      const double qmax = fmax(fmax(s_qmax[0], s_qmax[1]), fmax(s_qmax[2], s_qmax[3]));
+      const int ikl = k0 + lane;
+      bool lv = false;
+      if (lane < a.kchunk && ikl < a.nket && (!a.same || ikl <= klast)) {
+        lv = a.pairs[a.ket0 + ikl].Q * qmax >= a.tau;
+        if (!lv) {  // every bra lane of this wavefront that takes the ket is screened out
+          const int lo = a.same ? max(ikl, tile0) : tile0, hi = min(a.nbra, tile0 + kTileB);
+          nskip += hi > lo ? hi - lo : 0;
         }
       }
-      for (int s = 0; s < ns; ++s) {
-        double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
-        if (act) {
-          DBlocks<NA, NB, NC, ND> d;
-          d.load(D0 + s * nn, n, ma, mb, mc, md);
-          // target (row, ket column c < NK): LDS block or the global accumulator
-          auto put = [&](int row, int c, double x) {
-            if (lds)
-              unsafeAtomicAdd(&kbuf[row * NK + c], x);
-            else
-              unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
-          };
+      live = __ballot(lv);  // the same in every wavefront of the workgroup
+    }
+    while (live) {
+      const int ik = __builtin_amdgcn_readfirstlane(k0 + __builtin_ctzll(live));
+      live &= live - 1;
+      const Pair K = a.pairs[a.ket0 + ik];
+      bool act = ib < a.nbra && (!a.same || ik <= ib);
+      if (act && P.Q * K.Q < a.tau) {
+        act = false;
+        ++nskip;
+      }
+      double v[NAB * NCD];
+      if (act) {
+        eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                  K.ABz, a.boys, v);
+        ++nq;
+        nprimq += (unsigned long long)P.npp * K.npp;
+      } else {
+#pragma unroll
+        for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
+      }
+      const bool wact = __any(act);  // wave-uniform
+      const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * ((ib != ik || !a.same) ? 2.0 : 1.0)
+                             : 0.0;
+      // inactive lanes read a valid (ket) block and contribute zero
+      const int ma = act ? a.shells[P.A].ao : a.shells[K.A].ao, mb = act ? a.shells[P.B].ao : a.shells[K.B].ao;
+      const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+      const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+      for (int w = 0; w < a.nwfn; ++w) {
+        const double* D0 = a.dens + (size_t)w * dstride * nn;
+        if (wact) {
+          double rab[NAB], rcd[NCD];
 #pragma unroll
           for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
-#pragma unroll
-              for (int j = 0; j < NB; ++j)
-#pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
-              put(ma + i, k, fk * t);
+            for (int j = 0; j < NB; ++j) {
+              double t = D0[(size_t)(ma + i) * n + mb + j];
+              if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
+              rab[i * NB + j] = cfac * t;
             }
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < NA; ++i)
-#pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
-              put(mb + j, k, fk * t);
-            }
-#pragma unroll
-          for (int i = 0; i < NA; ++i)
+          for (int k = 0; k < NC; ++k)
 #pragma unroll
             for (int l = 0; l < ND; ++l) {
+              double t = D0[(size_t)(mc + k) * n + md + l];
+              if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
+              rcd[k * ND + l] = cfac * t;
+            }
+          double* J = a.jacc + (size_t)w * nn;
+          if (act) {
+#pragma unroll
+            for (int ij = 0; ij < NAB; ++ij) {
               double t = 0.0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
-              put(ma + i, NC + l, fk * t);
-            }
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < NA; ++i)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
-              put(mb + j, NC + l, fk * t);
-            }
-        }
-        if (lds) {
-          __syncthreads();
-          for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
-            const double x = kbuf[e];
-            if (x != 0.0) {
-              kbuf[e] = 0.0;
-              const int row = e / NK, c = e - row * NK;
-              unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
+              for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+              if constexpr (kRegJ) {
+                if (racc) {
+                  jreg[ij] += fj * t;
+                  continue;
+                }
+              }
+              unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], fj * t);
             }
           }
-          __syncthreads();
+#pragma unroll
+          for (int kl = 0; kl < NCD; ++kl) {
+            double t = 0.0;
+#pragma unroll
+            for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
+            t = wave_sum(fj * t);
+            if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], t);
+          }
+        }
+        for (int s = 0; s < ns; ++s) {
+          double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+          if (act) {
+            DBlocks<NA, NB, NC, ND> d;
+            d.load(D0 + s * nn, n, ma, mb, mc, md);
+            // target (row, ket column c < NK): LDS block or the global accumulator
+            auto put = [&](int row, int c, double x) {
+              if (lds)
+                unsafeAtomicAdd(&kbuf[row * NK + c], x);
+              else
+                unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
+            };
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
+                put(ma + i, k, fk * t);
+              }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
+                put(mb + j, k, fk * t);
+              }
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
+                put(ma + i, NC + l, fk * t);
+              }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
+                put(mb + j, NC + l, fk * t);
+              }
+          }
+          if (lds) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
+              const double x = kbuf[e];
+              if (x != 0.0) {
+                kbuf[e] = 0.0;
+                const int row = e / NK, c = e - row * NK;
+                unsafeAtomicAdd(&Kx[(size_t)row * n + (c < NC ? mc + c : md + c - NC)], x);
+              }
+            }
+            __syncthreads();
+          }
         }
       }
     }
@@ -493,6 +538,7 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
   constexpr int NAB = NA * NB, NCD = NC * ND, NR = NA + NB;
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
+  __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
   const int n = a.nbf;
   const size_t nn = (size_t)n * n;
   const int ns = a.nspin;
@@ -525,13 +571,16 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
       jreg[kl] = 0.0;
     }
   };
+  // A claim takes a chunk of kchunk consecutive bra pairs; with more than one,
+  // bra pairs that fail the Schwarz test against the strip's largest ket bound
+  // are skipped without a visit.
   int st = (int)(blockIdx.x % (unsigned)a.nstrip), left = a.nstrip;
   for (;;) {
-    __syncthreads();  // the previous unit is flushed and s_unit is free
-    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, 1);
+    __syncthreads();  // the previous unit is flushed and s_unit, s_qmax are free
+    if (threadIdx.x == 0) s_unit = atomicAdd(a.tile_ctr + st, a.kchunk);
     __syncthreads();
-    const int ib = __builtin_amdgcn_readfirstlane(s_unit);
-    if (ib >= a.nbra) {  // this ket strip is exhausted
+    const int b0 = __builtin_amdgcn_readfirstlane(s_unit);
+    if (b0 >= a.nbra) {  // this ket strip is exhausted
       if (--left == 0) break;
       st = st + 1 == a.nstrip ? 0 : st + 1;
       continue;
@@ -540,144 +589,167 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
       flush_j(jst);
       jst = st;
     }
-    const int ik = st * kStripK + wave * kTileB + lane;
-    const Pair P = a.pairs[a.bra0 + ib];
-    bool act = ik < a.nket;
+    const int tile0 = st * kStripK + wave * kTileB;
+    const int ik = tile0 + lane;
+    const bool kin = ik < a.nket;
     // every lane holds a pair of the ket class (the last one past the end)
-    const Pair K = a.pairs[a.ket0 + (act ? ik : a.nket - 1)];
-    if (act && P.Q * K.Q < a.tau) {
-      act = false;
-      ++nskip;
-    }
-    double v[NAB * NCD];
-    if (act) {
-      eri_block_ub<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                   K.ABz, a.boys, v);
-      ++nq;
-      nprimq += (unsigned long long)P.npp * K.npp;
-    } else {
-#pragma unroll
-      for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
-    }
-    const bool wact = __any(act);  // wave-uniform
-    // bra and ket classes differ, so every quartet stands for its (cd|ab) image too
-    const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * 2.0 : 0.0;
-    const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
-    const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
-    const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
-    for (int w = 0; w < a.nwfn; ++w) {
-      const double* D0 = a.dens + (size_t)w * dstride * nn;
-      if (wact) {
-        double rab[NAB], rcd[NCD];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            double t = D0[(size_t)(ma + i) * n + mb + j];
-            if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
-            rab[i * NB + j] = cfac * t;
-          }
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-#pragma unroll
-          for (int l = 0; l < ND; ++l) {
-            double t = D0[(size_t)(mc + k) * n + md + l];
-            if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
-            rcd[k * ND + l] = cfac * t;
-          }
-        double* J = a.jacc + (size_t)w * nn;
-        if (act) {
-#pragma unroll
-          for (int kl = 0; kl < NCD; ++kl) {
-            double t = 0.0;
-#pragma unroll
-            for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
-            if constexpr (kRegJ) {
-              if (racc) {
-                jreg[kl] += fj * t;
-                continue;
-              }
-            }
-            unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], fj * t);
-          }
-        }
-#pragma unroll
-        for (int ij = 0; ij < NAB; ++ij) {
-          double t = 0.0;
-#pragma unroll
-          for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
-          t = wave_sum(fj * t);
-          if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], t);
+    const Pair K = a.pairs[a.ket0 + (kin ? ik : a.nket - 1)];
+    unsigned long long live = 1;
+    if (a.kchunk > 1) {
+      const double qw = wave_max(kin ? K.Q : 0.0);
+      if (lane == 0) s_qmax[wave] = qw;
+      __syncthreads();
+      const double qmax = fmax(fmax(s_qmax[0], s_qmax[1]), fmax(s_qmax[2], s_qmax[3]));
+      const int ibl = b0 + lane;
+      bool lv = false;
+      if (lane < a.kchunk && ibl < a.nbra) {
+        lv = a.pairs[a.bra0 + ibl].Q * qmax >= a.tau;
+        if (!lv) {  // every ket lane of this wavefront is screened out
+          const int hi = min(a.nket, tile0 + kTileB);
+          nskip += hi > tile0 ? hi - tile0 : 0;
         }
       }
-      for (int s = 0; s < ns; ++s) {
-        double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
-        if (act) {
-          DBlocks<NA, NB, NC, ND> d;
-          d.load(D0 + s * nn, n, ma, mb, mc, md);
-          // target (bra row r < NR, column col): LDS block or the global accumulator
-          auto put = [&](int r, int col, double x) {
-            if (lds)
-              unsafeAtomicAdd(&kbuf[r * n + col], x);
-            else
-              unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
-          };
+      live = __ballot(lv);  // the same in every wavefront of the workgroup
+    }
+    while (live) {
+      const int ib = __builtin_amdgcn_readfirstlane(b0 + __builtin_ctzll(live));
+      live &= live - 1;
+      const Pair P = a.pairs[a.bra0 + ib];
+      bool act = kin;
+      if (act && P.Q * K.Q < a.tau) {
+        act = false;
+        ++nskip;
+      }
+      double v[NAB * NCD];
+      if (act) {
+        eri_block_ub<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
+                                     K.ABz, a.boys, v);
+        ++nq;
+        nprimq += (unsigned long long)P.npp * K.npp;
+      } else {
+#pragma unroll
+        for (int k = 0; k < NAB * NCD; ++k) v[k] = 0.0;
+      }
+      const bool wact = __any(act);  // wave-uniform
+      // bra and ket classes differ, so every quartet stands for its (cd|ab) image too
+      const double deg = act ? (P.offdiag ? 2.0 : 1.0) * (K.offdiag ? 2.0 : 1.0) * 2.0 : 0.0;
+      const int ma = a.shells[P.A].ao, mb = a.shells[P.B].ao;
+      const int mc = a.shells[K.A].ao, md = a.shells[K.B].ao;
+      const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
+      for (int w = 0; w < a.nwfn; ++w) {
+        const double* D0 = a.dens + (size_t)w * dstride * nn;
+        if (wact) {
+          double rab[NAB], rcd[NCD];
 #pragma unroll
           for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
-#pragma unroll
-              for (int j = 0; j < NB; ++j)
-#pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
-              put(i, mc + k, fk * t);
+            for (int j = 0; j < NB; ++j) {
+              double t = D0[(size_t)(ma + i) * n + mb + j];
+              if (ns == 2) t += D0[nn + (size_t)(ma + i) * n + mb + j];
+              rab[i * NB + j] = cfac * t;
             }
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < NA; ++i)
-#pragma unroll
-                for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
-              put(NA + j, mc + k, fk * t);
-            }
-#pragma unroll
-          for (int i = 0; i < NA; ++i)
+          for (int k = 0; k < NC; ++k)
 #pragma unroll
             for (int l = 0; l < ND; ++l) {
+              double t = D0[(size_t)(mc + k) * n + md + l];
+              if (ns == 2) t += D0[nn + (size_t)(mc + k) * n + md + l];
+              rcd[k * ND + l] = cfac * t;
+            }
+          double* J = a.jacc + (size_t)w * nn;
+          if (act) {
+#pragma unroll
+            for (int kl = 0; kl < NCD; ++kl) {
               double t = 0.0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
-              put(i, md + l, fk * t);
-            }
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < NA; ++i)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
-              put(NA + j, md + l, fk * t);
-            }
-        }
-        if (lds) {
-          __syncthreads();
-          for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
-            const double x = kbuf[e];
-            if (x != 0.0) {
-              kbuf[e] = 0.0;
-              const int r = e / n, col = e - r * n;
-              unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
+              for (int ij = 0; ij < NAB; ++ij) t += v[ij * NCD + kl] * rab[ij];
+              if constexpr (kRegJ) {
+                if (racc) {
+                  jreg[kl] += fj * t;
+                  continue;
+                }
+              }
+              unsafeAtomicAdd(&J[(size_t)(mc + kl / ND) * n + md + kl % ND], fj * t);
             }
           }
-          __syncthreads();
+#pragma unroll
+          for (int ij = 0; ij < NAB; ++ij) {
+            double t = 0.0;
+#pragma unroll
+            for (int kl = 0; kl < NCD; ++kl) t += v[ij * NCD + kl] * rcd[kl];
+            t = wave_sum(fj * t);
+            if (lane == 0 && t != 0.0) unsafeAtomicAdd(&J[(size_t)(ma + ij / NB) * n + mb + ij % NB], t);
+          }
+        }
+        for (int s = 0; s < ns; ++s) {
+          double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+          if (act) {
+            DBlocks<NA, NB, NC, ND> d;
+            d.load(D0 + s * nn, n, ma, mb, mc, md);
+            // target (bra row r < NR, column col): LDS block or the global accumulator
+            auto put = [&](int r, int col, double x) {
+              if (lds)
+                unsafeAtomicAdd(&kbuf[r * n + col], x);
+              else
+                unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
+            };
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.bd[j * ND + l];
+                put(i, mc + k, fk * t);
+              }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                  for (int l = 0; l < ND; ++l) t += v[((i * NB + j) * NC + k) * ND + l] * d.ad[i * ND + l];
+                put(NA + j, mc + k, fk * t);
+              }
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.bc[j * NC + k];
+                put(i, md + l, fk * t);
+              }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+              for (int l = 0; l < ND; ++l) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                  for (int k = 0; k < NC; ++k) t += v[((i * NB + j) * NC + k) * ND + l] * d.ac[i * NC + k];
+                put(NA + j, md + l, fk * t);
+              }
+          }
+          if (lds) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < kb_len; e += blockDim.x) {
+              const double x = kbuf[e];
+              if (x != 0.0) {
+                kbuf[e] = 0.0;
+                const int r = e / n, col = e - r * n;
+                unsafeAtomicAdd(&Kx[(size_t)(r < NA ? ma + r : mb + r - NA) * n + col], x);
+              }
+            }
+            __syncthreads();
+          }
         }
       }
     }

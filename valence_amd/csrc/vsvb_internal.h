@@ -100,8 +100,9 @@ struct ClassLaunch {
   int claim = 1;      // cooperative kernel: units per claim
   int per_cu = 0;     // workgroups per CU requested (0 = the kernel's occupancy)
   bool tr = false;    // straight-line J/K: transposed units (one bra pair x 256 ket pairs)
-  int strips = 1;     // straight-line J/K: bra strips (normal) or ket strips (transposed), one work counter each
-  int kchunk = 1;     // straight-line energy: ket pairs per unit
+  int strips = 1;     // straight-line: energy segments, J/K bra strips (ket strips transposed), one work counter each
+  int kchunk = 1;     // straight-line: pairs of the uniform side (ket; bra when transposed) per claim
+  int seg_tiles = 1;  // straight-line energy: bra tiles per counter segment
 };
 
 }  // namespace vsvb
@@ -120,7 +121,7 @@ struct vsvb_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (device entry)
   double tau = 1e-17;
-  int ket_chunk = 0;  // energy-mode ket pairs per unit (0: from the system size)
+  int pair_chunk = 0;  // straight-line units: uniform-side pairs per claim (0: from the system size)
   // sizes
   int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
   int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;
