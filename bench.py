@@ -94,6 +94,32 @@ def tuner_summary(t) -> dict:
     return out
 
 
+def lif_leg():
+    """LiF 4x4x8 slab (128 atoms, 1536 AOs, 384 docc): one energy and one energy +
+    gradient after a warm-up of each, with per-class device times (HIP events)."""
+    from valence_amd import lib as L
+    from valence_amd import workloads as W
+    s = W.config4()
+    out = {"workload": "LiF 4x4x8 slab, 128 atoms, 1536 AOs, 384 docc"}
+    with L.Context(s) as ctx:
+        ctx.set_profiling(True)
+        ctx.energy()
+        ctx.energy_grad()
+        _progress("lif leg: warm-up done")
+        for what, call in (("energy", ctx.energy), ("energy_grad", ctx.energy_grad)):
+            t0 = time.perf_counter()
+            r = call()
+            t = time.perf_counter() - t0
+            st = ctx.class_stats()
+            cnt = ctx.counters()
+            out[what] = {"seconds": t, "energy": float(r if what == "energy" else r[0]),
+                         "quartet_stage_ms": float(st[:, 0].sum()), "quartets": float(cnt[0]), "skipped": float(cnt[2]),
+                         "quartet_stage_tflops": float(st[:, 4].sum() / (st[:, 0].sum() * 1e-3) / 1e12),
+                         "class_ms": {CLASS_NAMES[q]: float(st[q, 0]) for q in range(21)}}
+            _progress(f"lif leg: {what} {t:.3f} s")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,9 +131,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="profiling aid: enqueue every batch run directly instead of replaying a HIP graph")
     ap.add_argument("--class-stats", default="", help="write per-class statistics (JSON) to this path")
-    ap.add_argument("--leg", choices=["both", "plain", "batch", "copy"], default="both",
+    ap.add_argument("--leg", choices=["both", "plain", "batch", "copy", "lif"], default="both",
                     help="profiling aid: run one leg only (no JSON line unless both); 'copy' runs a 1 GiB "
-                         "device copy (torch) used to calibrate the FETCH_SIZE/WRITE_SIZE counters")
+                         "device copy (torch) used to calibrate the FETCH_SIZE/WRITE_SIZE counters; 'lif' "
+                         "times one energy and one energy + gradient of the 128-atom LiF slab")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU; the path shards by "
@@ -128,6 +155,10 @@ def main():
     from valence_amd import lib as L
     from valence_amd import workloads as W
     L.load()
+
+    if args.leg == "lif":
+        lif_leg()
+        return
 
     plain = W.config1()
     systems = W.config3()

@@ -828,7 +828,9 @@ constexpr int kDefaultJkPerCu = 4;
 // strips keep more lanes busy than the bra strips (few bra pairs), or, at equal
 // lane use, when the bra block is the larger one (fewer per-lane atomics); but
 // not into an orientation whose exchange rows no longer fit in LDS (large
-// basis sets) while the other's do.
+// basis sets) while the other's do, and for large basis sets, at equal lane
+// use, the one with fewer exchange rows.
+constexpr int kLargeBasis = 1024;
 static bool transposed_default(vsvb_context* c, int bc, int kc) {
   if (bc == kc) return false;
   const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
@@ -838,11 +840,15 @@ static bool transposed_default(vsvb_context* c, int bc, int kc) {
   const int nab = ncart(kPairLs[bc][0]) * ncart(kPairLs[bc][1]), ncd = ncart(kPairLs[kc][0]) * ncart(kPairLs[kc][1]);
   // exchange staging: the normal unit holds NC + ND rows of nbf, the transposed one NA + NB
   const size_t row = (size_t)c->nbf * sizeof(double);
-  const bool fit_n = row * (ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1])) <= kMaxLdsK;
-  const bool fit_t = row * (ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1])) <= kMaxLdsK;
+  const int rows_n = ncart(kPairLs[kc][0]) + ncart(kPairLs[kc][1]);
+  const int rows_t = ncart(kPairLs[bc][0]) + ncart(kPairLs[bc][1]);
+  const bool fit_n = row * rows_n <= kMaxLdsK, fit_t = row * rows_t <= kMaxLdsK;
   if (uk > ub + 0.05) return fit_t || !fit_n;
   if (ub > uk + 0.05) return false;
   if (fit_n != fit_t) return fit_t;
+  // large basis: the orientation with fewer exchange rows (LiF128's (ps|ss):
+  // normal 219 ms against transposed 400 ms, as the batch tuner measured it)
+  if (c->nbf >= kLargeBasis && rows_n != rows_t) return rows_t < rows_n;
   return nab > ncd;
 }
 
