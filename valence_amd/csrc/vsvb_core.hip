@@ -1075,6 +1075,10 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
+      // work units are 32-bit indices (the cooperative kernel's, one per pair x pair, are the most)
+      const long long nbk = (long long)(c->pair_off[bc + 1] - c->pair_off[bc]) * (c->pair_off[kc + 1] - c->pair_off[kc]);
+      if (nbk > (long long)INT32_MAX - (1LL << 24))
+        return fail(VSVB_ERR_UNSUPPORTED, "shell-pair classes too large for 32-bit work-unit indices");
       c->coop_idx[q] = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
       const bool coop = coop_default(c->coop_idx[q]);
       const bool tr = transposed_default(c, bc, kc);
