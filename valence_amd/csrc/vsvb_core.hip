@@ -772,6 +772,9 @@ static int class_bra(int q) {
 // as many workgroups as the kernel's occupancy allows (energy: at most 2048,
 // measured fastest for the low-AM classes).
 constexpr double kEnergyUnitsPerWave = 32.0;
+// Work units are 32-bit indices; counters may run past the last unit by at
+// most a claim per workgroup, which the margin covers.
+constexpr long long kMaxUnits = (long long)INT32_MAX - (1LL << 24);
 constexpr double kJkUnitsPerBlock = 16.0;
 
 static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int per_cu, bool lds_on, bool tr) {
@@ -790,7 +793,9 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
     while (c->pair_chunk == 0 && kch < 64 && nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap)
       kch *= 2;
     L.kchunk = kch;
-    L.units = (int)nbt * ((nk + kch - 1) / kch);
+    const long long units = (long long)nbt * ((nk + kch - 1) / kch);
+    L.overflow = units > kMaxUnits;
+    L.units = L.overflow ? 0 : (int)units;
     // several counter segments once claims are chunked (large systems: many
     // short claims); one counter, heaviest tiles first, otherwise
     L.seg_tiles = kch > 1 ? kSegTiles : (int)nbt;
@@ -813,7 +818,9 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
          (double)L.strips * ((side + 2 * kch - 1) / (2 * kch)) >= kJkUnitsPerBlock * cap)
     kch *= 2;
   L.kchunk = kch;
-  L.units = L.strips * ((side + kch - 1) / kch);
+  const long long units = (long long)L.strips * ((side + kch - 1) / kch);
+  L.overflow = units > kMaxUnits;
+  L.units = L.overflow ? 0 : (int)units;
   L.blocks = std::min(L.units, cap);
   return L;
 }
@@ -863,6 +870,8 @@ static ClassLaunch coop_launch(vsvb_context* c, int q, bool jk, int per_cu, int 
   const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
   const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
   if (nb <= 0 || nk <= 0 || c->coop_idx[q] < 0) return L;  // an empty class launches nothing
+  L.overflow = (long long)nb * nk > kMaxUnits;
+  if (L.overflow) return L;
   L.units = nb * nk;
   const int occ = coop_occupancy(c->coop_idx[q], jk);
   L.blocks = std::min(L.units, (per_cu > 0 ? std::min(per_cu, occ) : occ) * c->ncu);
@@ -1075,10 +1084,6 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
-      // work units are 32-bit indices (the cooperative kernel's, one per pair x pair, are the most)
-      const long long nbk = (long long)(c->pair_off[bc + 1] - c->pair_off[bc]) * (c->pair_off[kc + 1] - c->pair_off[kc]);
-      if (nbk > (long long)INT32_MAX - (1LL << 24))
-        return fail(VSVB_ERR_UNSUPPORTED, "shell-pair classes too large for 32-bit work-unit indices");
       c->coop_idx[q] = coop_class(kPairLs[bc][0], kPairLs[bc][1], kPairLs[kc][0], kPairLs[kc][1]);
       const bool coop = coop_default(c->coop_idx[q]);
       const bool tr = transposed_default(c, bc, kc);
@@ -1087,6 +1092,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       c->le[q] = coop_e ? coop_launch(c, q, false, 0, 0) : straight_launch(c, bc, kc, false, 0, false, false);
       c->ljk[q] = coop_jk ? coop_launch(c, q, true, 0, 0)
                           : straight_launch(c, bc, kc, true, kDefaultJkPerCu, true, tr);
+      if (c->le[q].overflow || c->ljk[q].overflow)
+        return fail(VSVB_ERR_UNSUPPORTED, "shell-pair classes too large for 32-bit work-unit indices");
       c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
@@ -1711,6 +1718,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       double tbest = 1e30, tdef = 0.0;
       for (size_t k = 0; k < cand.size(); ++k) {
         const ClassLaunch cl = policy_launch(L, q, cand[k]);
+        if (cl.overflow) continue;  // not a launch this class can take
         const QuartetArgs a = class_args(L, bc, kc, cl, true, nwfn, nspin, dens, jacc, kacc, L->d_partial.p);
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {

@@ -103,6 +103,7 @@ struct ClassLaunch {
   int strips = 1;     // straight-line: energy segments, J/K bra strips (ket strips transposed), one work counter each
   int kchunk = 1;     // straight-line: pairs of the uniform side (ket; bra when transposed) per claim
   int seg_tiles = 1;  // straight-line energy: bra tiles per counter segment
+  bool overflow = false;  // the class has more work units than 32-bit indices allow in this shape
 };
 
 }  // namespace vsvb
