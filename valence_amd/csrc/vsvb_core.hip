@@ -1526,11 +1526,12 @@ struct BatchGroup {
 // at creation, so the captured work never changes.
 constexpr int kTuneBuckets = 2;  // groups of one wave function, groups of several
 constexpr int kTuneCols = 8;
-constexpr int kBatchLanes = 4;  // streams the groups of a batch run on, round-robin
+constexpr int kBatchLanes = 4;  // streams the groups of a batch run on
 
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
   std::vector<BatchGroup> groups;
+  std::vector<int> order;                // groups in enqueue order (estimated cost, largest first)
   hipStream_t stream = nullptr;          // the run's own stream: fork, join, gather, graph launch
   std::vector<hipStream_t> lanes;        // group streams (default priority), forked from and joined to stream
   hipEvent_t ev_fork = nullptr;
@@ -1568,6 +1569,31 @@ struct vsvb_batch {
   }
 };
 
+// Groups go to the lane streams longest first, each to the lane with the least
+// estimated work so far, and every lane enqueues its groups in that order, so
+// that one large group (many densities on one molecule) does not end up
+// behind a lane's share of small ones.  Estimated work of a group: squared
+// shell-pair count (quartets) times 1 + (densities - 1) / 5 (the quartet
+// pass of 32 C8H18 densities measured about 7x that of one).
+static void assign_lanes(vsvb_batch* b, int nl) {
+  const size_t ng = b->groups.size();
+  std::vector<double> cost(ng);
+  for (size_t k = 0; k < ng; ++k) {
+    const vsvb_context* lead = b->ctx[b->groups[k].members[0]];
+    const double np = lead->npairs;
+    cost[k] = np * np * (1.0 + 0.2 * ((double)b->groups[k].members.size() - 1.0));
+  }
+  b->order.resize(ng);
+  for (size_t k = 0; k < ng; ++k) b->order[k] = (int)k;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<double> load(nl, 0.0);
+  for (int gi : b->order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[l] += cost[gi];
+    for (int i : b->groups[gi].members) b->ctx[i]->stream = b->lanes[l];
+  }
+}
+
 constexpr int kCnt = 4 * kNQuartetClass;  // counters per context
 
 // All device work of one batch run, no host synchronization.  The groups run
@@ -1579,7 +1605,8 @@ static int batch_enqueue(vsvb_batch* b) {
   for (auto s : b->lanes) HIP_OK(hipStreamWaitEvent(s, b->ev_fork, 0));
   // Every wave function goes through the J/K (adjoint-ready) contraction; the
   // energy is assembled from it, and the gradient when asked for.
-  for (auto& g : b->groups) {
+  for (int gi : b->order) {
+    BatchGroup& g = b->groups[gi];
     vsvb_context* lead = b->ctx[g.members[0]];
     for (int i : g.members)
       for (int k = 0; k < b->ctx[i]->totlen; ++k)
@@ -2250,7 +2277,7 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
         bt->groups.back().members.push_back(j);
       }
   }
-  // group streams: group k runs on lane k mod kBatchLanes
+  // group streams
   const int nl = std::min<int>(kBatchLanes, (int)bt->groups.size());
   HIP_OK(hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming));
   for (int k = 0; k < nl; ++k) {
@@ -2261,8 +2288,7 @@ int vsvb_batch_create(int32_t nwfn, const vsvb_system* const* systems, const vsv
     HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     bt->ev_join.push_back(e);
   }
-  for (size_t k = 0; k < bt->groups.size(); ++k)
-    for (int i : bt->groups[k].members) bt->ctx[i]->stream = bt->lanes[k % nl];
+  assign_lanes(bt, nl);
   for (auto& g : bt->groups) {
     vsvb_context* lead = bt->ctx[g.members[0]];
     if (g.members.size() < 2) continue;
@@ -2388,8 +2414,7 @@ int vsvb_batch_set_lanes(vsvb_batch* b, int32_t n) {
     HIP_OK(hipGraphExecDestroy(b->exec));
     b->exec = nullptr;
   }
-  for (size_t k = 0; k < b->groups.size(); ++k)
-    for (int i : b->groups[k].members) b->ctx[i]->stream = b->lanes[k % nl];
+  assign_lanes(b, nl);
   return VSVB_OK;
 }
 
