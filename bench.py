@@ -78,7 +78,7 @@ def tuner_summary(t) -> dict:
     """Bucket -> class -> chosen J/K launch ('coop', or 'lds'/'atomic' x workgroups per CU, 'transposed' when the
     units are ket strips x bra pairs) and its time vs the default configuration."""
     out = {}
-    for k, bucket in enumerate(["one wave function per group", "several wave functions per group"]):
+    for k, bucket in enumerate(["groups of one to four wave functions", "groups of more than four wave functions"]):
         d = {}
         for q in range(21):
             r = t[k, q]

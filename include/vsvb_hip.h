@@ -159,8 +159,9 @@ int vsvb_batch_uses_graph(vsvb_batch *batch);
  * For tools that time or trace dispatches one by one and do not follow graph
  * launches; results are the same either way. */
 int vsvb_batch_set_graph(vsvb_batch *batch, int32_t on);
-/* Groups of a batch run on n streams (1..4, default 4; more is clipped), group
- * k on stream k mod n, so that independent groups overlap on the GPU.  An
+/* Groups of a batch run on n streams (1..4, default 4; more is clipped), largest
+ * estimated work first, each to the least-loaded stream, so that independent
+ * groups overlap on the GPU.  An
  * existing graph is released (the next run captures again unless graphs are
  * off).  n = 1 serializes the groups, for per-kernel timing without overlap.
  * No counterpart in the reference. */
@@ -175,8 +176,8 @@ int vsvb_batch_set_coords(vsvb_batch *batch, const double *coords);
  * vsvb_batch_set_coords, v_cm1[nwfn] energies in cm^-1. */
 int vsvb_surface_batch(vsvb_batch *batch, const double *x_angstrom, double *v_cm1);
 /* Launch choices of the batch tuner, which runs before the first run, for
- * groups of one wave function (bucket 0) and groups of several sharing one
- * quartet pass (bucket 1): out[2][21][8] = per bucket and class (J/K mode)
+ * groups of one wave function (bucket 0, also applied to groups of up to four)
+ * and groups of more than four sharing one quartet pass (bucket 1): out[2][21][8] = per bucket and class (J/K mode)
  * {cooperative kernel 0/1, workgroups per CU (0 = occupancy), LDS exchange
  * staging 0/1, units per claim (0 = automatic), transposed units 0/1, best ms,
  * default-configuration ms, tuned 0/1}; zeros where nothing was tuned. */

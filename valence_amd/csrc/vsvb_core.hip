@@ -1524,7 +1524,7 @@ struct BatchGroup {
 // device arrays) into one HIP graph; later runs replay it: one graph launch
 // and one copy to pinned host memory per run.  The systems of a batch are fixed
 // at creation, so the captured work never changes.
-constexpr int kTuneBuckets = 2;  // groups of one wave function, groups of several
+constexpr int kTuneBuckets = 2;  // groups of one wave function (and of a few), groups of more
 constexpr int kTuneCols = 8;
 constexpr int kBatchLanes = 4;  // streams the groups of a batch run on
 
@@ -1658,11 +1658,11 @@ static int batch_gather(vsvb_batch* b) {
 }
 
 // Launch tuner of the batch (J/K mode), run before the first run.  Groups of
-// one wave function (one density per quartet pass) and groups of several
-// (J/K contractions of many densities per quartet) are tuned apart: for each
-// bucket a representative closed- or high-spin group leader -- single: the
-// molecule size that occurs most often; several: the group with the most
-// density work -- times every candidate configuration of every class on its
+// one wave function (one density per quartet pass) and groups of more than
+// kFewDensities (J/K contractions of many densities per quartet) are tuned
+// apart: for each bucket a representative closed- or high-spin group leader --
+// single: the molecule size that occurs most often; many: the group with the
+// most density work -- times every candidate configuration of every class on its
 // own data: the straight-line kernel with 1, 2, 4 or as many workgroups per CU
 // as fit, with or without the LDS exchange staging, in both orientations
 // (classes with different bra and ket classes), and the cooperative kernel
@@ -1778,9 +1778,14 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
   return VSVB_OK;
 }
 
+// Groups of up to this many densities take the single-density policies (their
+// launches behave like one density's: measured 245 -> 247 evaluations/s on the
+// 96-wave-function batch, whose alkanes come in groups of two).
+constexpr size_t kFewDensities = 4;
+
 static int tune_batch(vsvb_batch* b) {
   // representatives: single-density groups -- the most frequent molecule size
-  // (ties: the larger); multi-density groups -- the most nwfn * nbf^2
+  // (ties: the larger); groups of more than kFewDensities -- the most nwfn * nbf^2
   std::map<int, int> freq;
   for (auto& g : b->groups) {
     vsvb_context* l = b->ctx[g.members[0]];
@@ -1799,7 +1804,7 @@ static int tune_batch(vsvb_batch* b) {
     if (l->npair > 0) continue;
     if (g.members.size() == 1) {
       if (!rep[0] && l->nbf == nbf1) rep[0] = &g;
-    } else {
+    } else if (g.members.size() > kFewDensities) {
       const double w = (double)g.members.size() * l->nbf * l->nbf;
       if (w > wmax) {
         wmax = w;
@@ -1814,7 +1819,7 @@ static int tune_batch(vsvb_batch* b) {
     if (rc) return rc;
   }
   for (auto& g : b->groups) {
-    const int k = g.members.size() > 1 ? 1 : 0;
+    const int k = g.members.size() > kFewDensities ? 1 : 0;
     if (!rep[k]) continue;
     for (int i : g.members) {
       vsvb_context* c = b->ctx[i];
