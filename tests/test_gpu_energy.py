@@ -266,3 +266,23 @@ def test_pair_chunk_gradient_matches(lib):
             assert abs(e - ref[0]) < 1e-9
             assert np.linalg.norm(g - ref[1]) <= 1e-10 * np.linalg.norm(ref[1])
             assert (st[0], st[1], st[2]) == ref[2:]
+
+
+def test_batch_many_density_group(lib):
+    """A group of more than four densities (six rotations of C8H18) runs the
+    launch policies tuned for many densities; every member must still match its
+    own independent evaluation."""
+    systems = workloads.config3()
+    pick = [52, 53, 54, 55, 56, 57, 3]
+    b = lib.Batch([systems[i] for i in pick])
+    e, g = b.run(want_grad=True)
+    t = b.tuning()
+    assert t[1, :, 7].sum() > 0          # the many-density bucket was tuned
+    e2, g2 = b.run(want_grad=True)       # replay of the captured graph
+    for k, i in enumerate(pick):
+        with lib.Context(systems[i]) as ctx:
+            e1, g1 = ctx.energy_grad()
+        print(f"wfn {i}: batch {e[k]:.12f} single {e1:.12f}")
+        assert abs(e[k] - e1) < 1e-9 and abs(e2[k] - e1) < 1e-9
+        assert np.linalg.norm(g[k] - g1) <= 1e-10 * np.linalg.norm(g1)
+        assert np.linalg.norm(g2[k] - g1) <= 1e-10 * np.linalg.norm(g1)
