@@ -476,35 +476,50 @@ __global__ void __launch_bounds__(64) k_gemm(int M, int N, int K, double alpha, 
 }
 
 // Small products (the orbital algebra of one wave function: tens of thousands
-// of multiply-adds per output tile at most) are latency-bound: there 256
-// threads per 16 x 16 tile with vector FMAs finish sooner (9 us against 16 us
-// per call, measured); FP64 vector and matrix peaks are equal on this chip.
-__global__ void __launch_bounds__(256) k_gemm_small(int M, int N, int K, double alpha, const double* __restrict__ A,
-                                                    int lda, int ta, const double* __restrict__ B, int ldb, int tb,
-                                                    double beta, double* __restrict__ C, int ldc) {
-  __shared__ double As[16][17], Bs[16][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+// of multiply-adds per output tile at most) are latency-bound: there vector
+// FMAs finish sooner than the matrix-core tiles (FP64 vector and matrix peaks
+// are equal on this chip).  A workgroup of kGemmSlices x 256 threads takes one
+// 16 x 16 tile: slice s accumulates the 16-wide k-steps s, s + kGemmSlices, ...
+// (each thread one output element), with the next step's operands loaded into
+// registers while the current one is multiplied, and the slices' partial sums
+// are added in LDS at the end.
+constexpr int kGemmSlices = 4;
+
+__global__ void __launch_bounds__(256 * kGemmSlices)
+    k_gemm_small(int M, int N, int K, double alpha, const double* __restrict__ A, int lda, int ta,
+                 const double* __restrict__ B, int ldb, int tb, double beta, double* __restrict__ C, int ldc) {
+  __shared__ double As[kGemmSlices][16][17], Bs[kGemmSlices][16][17];
+  __shared__ double part[kGemmSlices - 1][256];
+  const int sl = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int tx = t & 15, ty = t >> 4;
   const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  auto load_a = [&](int k0) {
+    const int k = k0 + tx;
+    return (row < M && k < K) ? (ta ? A[(size_t)k * lda + row] : A[(size_t)row * lda + k]) : 0.0;
+  };
+  auto load_b = [&](int k0) {
+    const int k = k0 + ty;
+    return (k < K && col < N) ? (tb ? B[(size_t)col * ldb + k] : B[(size_t)k * ldb + col]) : 0.0;
+  };
+  const int steps = (K + 16 * kGemmSlices - 1) / (16 * kGemmSlices);  // the same count in every slice
   double acc = 0.0;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    {
-      const int r = blockIdx.y * 16 + ty, k = k0 + tx;
-      double v = 0.0;
-      if (r < M && k < K) v = ta ? A[(size_t)k * lda + r] : A[(size_t)r * lda + k];
-      As[ty][tx] = v;
-    }
-    {
-      const int k = k0 + ty, c = blockIdx.x * 16 + tx;
-      double v = 0.0;
-      if (k < K && c < N) v = tb ? B[(size_t)c * ldb + k] : B[(size_t)k * ldb + c];
-      Bs[ty][tx] = v;
-    }
+  double va = load_a(16 * sl), vb = load_b(16 * sl);
+  for (int it = 0; it < steps; ++it) {
+    As[sl][ty][tx] = va;
+    Bs[sl][ty][tx] = vb;
     __syncthreads();
+    const int kn = 16 * (sl + kGemmSlices * (it + 1));  // next step of this slice (zeros past K)
+    va = load_a(kn);
+    vb = load_b(kn);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    for (int k = 0; k < 16; ++k) acc += As[sl][ty][k] * Bs[sl][k][tx];
     __syncthreads();
   }
-  if (row < M && col < N) {
+  if (sl > 0) part[sl - 1][t] = acc;
+  __syncthreads();
+  if (sl == 0 && row < M && col < N) {
+#pragma unroll
+    for (int s = 0; s < kGemmSlices - 1; ++s) acc += part[s][t];
     double* c = C + (size_t)row * ldc + col;
     *c = alpha * acc + (beta == 0.0 ? 0.0 : beta * *c);
   }
@@ -517,8 +532,8 @@ hipError_t vsvb::gemm(hipStream_t st, int M, int N, int K, double alpha, const d
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 g((N + 15) / 16, (M + 15) / 16);
   if ((double)M * N * K < kGemmMfmaMin)
-    hipLaunchKernelGGL(k_gemm_small, g, dim3(256), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C,
-                       ldc);
+    hipLaunchKernelGGL(k_gemm_small, g, dim3(256 * kGemmSlices), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb,
+                       (int)tb, beta, C, ldc);
   else
     hipLaunchKernelGGL(k_gemm, g, dim3(64), 0, st, M, N, K, alpha, A, lda, (int)ta, B, ldb, (int)tb, beta, C, ldc);
   return hipGetLastError();
