@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256 * kGemmSlices)
     k_gemm_small(int M, int N, int K, double alpha, const double* __restrict__ A, int lda, int ta,
                  const double* __restrict__ B, int ldb, int tb, double beta, double* __restrict__ C, int ldc) {
   __shared__ double As[kGemmSlices][16][17], Bs[kGemmSlices][16][17];
-  __shared__ double part[kGemmSlices - 1][256];
+  __shared__ double part[kGemmSlices > 1 ? kGemmSlices - 1 : 1][256];
   const int sl = threadIdx.x >> 8, t = threadIdx.x & 255;
   const int tx = t & 15, ty = t >> 4;
   const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
