@@ -547,7 +547,9 @@ __global__ void k_copy_block(const double* __restrict__ src, int ld, int off, in
 }
 
 // In-place Gauss-Jordan inverse of an SPD matrix (no pivoting needed: every
-// leading block of an SPD matrix has positive diagonal).  One workgroup.
+// leading block of an SPD matrix has positive diagonal).  One workgroup, for
+// matrices of up to kSpdOneBlock rows.
+constexpr int kSpdOneBlock = 128;
 __global__ void __launch_bounds__(1024) k_spd_inverse(double* __restrict__ a, int n, int* __restrict__ status) {
   extern __shared__ double colk[];
   for (int k = 0; k < n; ++k) {
@@ -563,16 +565,58 @@ __global__ void __launch_bounds__(1024) k_spd_inverse(double* __restrict__ a, in
     // scale row k
     for (int j = threadIdx.x; j < n; j += blockDim.x) a[(size_t)k * n + j] = (j == k ? 1.0 : a[(size_t)k * n + j]) * ip;
     __syncthreads();
-    const size_t nn = (size_t)n * n;
-    for (size_t t = threadIdx.x; t < nn; t += blockDim.x) {
-      const int i = (int)(t / n), j = (int)(t % n);
+    // rows by thread / 64 (16 at a time), columns by lane: no index division
+    for (int i = threadIdx.x >> 6; i < n; i += blockDim.x >> 6) {
       if (i == k) continue;
       const double f = colk[i];
-      const double akj = a[(size_t)k * n + j];
-      a[t] = (j == k ? 0.0 : a[t]) - f * akj;
+      for (int j = threadIdx.x & 63; j < n; j += 64) {
+        double* aij = a + (size_t)i * n + j;
+        *aij = (j == k ? 0.0 : *aij) - f * a[(size_t)k * n + j];
+      }
     }
     __syncthreads();
   }
+}
+
+// The same Gauss-Jordan elimination for large matrices, one pivot per pair of
+// launches so that the whole GPU takes each step (one workgroup would be bound
+// by one CU's cache bandwidth: 20 ms for n = 384).  Step k: k_gj_pivot saves
+// column k and scales row k; k_gj_update eliminates column k from every other
+// row.
+__global__ void k_gj_pivot(double* __restrict__ a, int n, int k, double* __restrict__ colk, int* __restrict__ status) {
+  const double piv = a[(size_t)k * n + k];
+  if (!(piv > 0.0) || !isfinite(piv)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *status = 1;
+    return;
+  }
+  const double ip = 1.0 / piv;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    colk[i] = a[(size_t)i * n + k];
+    a[(size_t)k * n + i] = (i == k ? 1.0 : a[(size_t)k * n + i]) * ip;
+  }
+}
+
+__global__ void k_gj_update(double* __restrict__ a, int n, int k, const double* __restrict__ colk) {
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= n || j >= n || i == k) return;
+  double* aij = a + (size_t)i * n + j;
+  *aij = (j == k ? 0.0 : *aij) - colk[i] * a[(size_t)k * n + j];
+}
+
+// In-place inverse of an SPD matrix: one workgroup for small n, a pair of
+// launches per pivot for large n.  colk: n doubles of scratch.
+static hipError_t launch_spd_inverse(hipStream_t st, double* a, int n, int* status, double* colk) {
+  if (n <= kSpdOneBlock) {
+    hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), n * sizeof(double), st, a, n, status);
+    return hipGetLastError();
+  }
+  const dim3 g((n + 63) / 64, (n + 3) / 4);
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3((n + 255) / 256), dim3(256), 0, st, a, n, k, colk, status);
+    hipLaunchKernelGGL(k_gj_update, g, dim3(256), 0, st, a, n, k, colk);
+  }
+  return hipGetLastError();
 }
 
 // out[0] += w * sum_ij A[off+i][off+j] * B[i][j]  (A ld-strided block, B n x n)
@@ -1054,6 +1098,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(c->d_ao_shell.alloc(c->nbf));
   HIP_OK(c->d_orb_start.alloc(s->norbs + 1));
   HIP_OK(c->d_status.alloc(4));
+  HIP_OK(c->d_colk.alloc(std::max(1, c->nsp)));
   HIP_OK(c->d_ent.alloc(c->ent.size()));
   HIP_OK(c->d_coeff.alloc(std::max(1, s->totlen)));
   HIP_OK(c->d_ndf_scale.alloc(std::max(1, s->ndf)));
@@ -1257,8 +1302,7 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
     double* P = c->d_P.p + (size_t)s * ns * ns;
     if (m == 0) continue;
     hipLaunchKernelGGL(k_copy_block, dim3((m * m + 255) / 256), dim3(256), 0, st, c->d_Sorb.p, ns, off, m, P);
-    hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), m * sizeof(double), st, P, m, c->d_status.p);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_spd_inverse(st, P, m, c->d_status.p, c->d_colk.p));
     // E1 += w tr(h_s P_s): closed shell counts both spins (w = 2)
     hipLaunchKernelGGL(k_trace_prod, dim3(1), dim3(256), 0, st, c->d_Horb.p, ns, off, P, m, nspin == 1 ? 2.0 : 1.0,
                        c->d_scal.p + 0);
