@@ -79,6 +79,9 @@ def test_lif128_invariances(lib, lif128):
         st = ctx.counters()
         print(f"E={e0:.10f} quartets={st[0]:.0f} skipped={st[2]:.0f} eri_ms={st[3]:.2f}", flush=True)
         assert int(st[0]) + int(st[2]) == _unique_quartets(s)
+        # repeated calls (the 384 x 384 overlap inverse runs one pivot per pair of grid-wide launches)
+        for _ in range(3):
+            assert abs(ctx.energy() - e0) < _tol(e0)
         # scale 40 random orbitals
         c = s.coeff.copy()
         for p in rng.choice(s.norbs, 40, replace=False):

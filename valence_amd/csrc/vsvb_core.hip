@@ -581,8 +581,10 @@ __global__ void __launch_bounds__(1024) k_spd_inverse(double* __restrict__ a, in
 // The same Gauss-Jordan elimination for large matrices, one pivot per pair of
 // launches so that the whole GPU takes each step (one workgroup would be bound
 // by one CU's cache bandwidth: 20 ms for n = 384).  Step k: k_gj_pivot saves
-// column k and scales row k; k_gj_update eliminates column k from every other
-// row.
+// column k (pivot included) and scales row k off the diagonal; k_gj_update
+// eliminates column k from every other row and stores the pivot's inverse.
+// The pivot element itself is written only in k_gj_update, after every
+// workgroup of k_gj_pivot has read it.
 __global__ void k_gj_pivot(double* __restrict__ a, int n, int k, double* __restrict__ colk, int* __restrict__ status) {
   const double piv = a[(size_t)k * n + k];
   if (!(piv > 0.0) || !isfinite(piv)) {
@@ -592,16 +594,22 @@ __global__ void k_gj_pivot(double* __restrict__ a, int n, int k, double* __restr
   const double ip = 1.0 / piv;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     colk[i] = a[(size_t)i * n + k];
-    a[(size_t)k * n + i] = (i == k ? 1.0 : a[(size_t)k * n + i]) * ip;
+    if (i != k) a[(size_t)k * n + i] *= ip;
   }
 }
 
 __global__ void k_gj_update(double* __restrict__ a, int n, int k, const double* __restrict__ colk) {
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= n || j >= n || i == k) return;
+  if (i >= n || j >= n) return;
+  const double ip = 1.0 / colk[k];
   double* aij = a + (size_t)i * n + j;
-  *aij = (j == k ? 0.0 : *aij) - colk[i] * a[(size_t)k * n + j];
+  if (i == k) {
+    if (j == k) *aij = ip;  // row k is otherwise final
+    return;
+  }
+  const double akj = j == k ? ip : a[(size_t)k * n + j];
+  *aij = (j == k ? 0.0 : *aij) - colk[i] * akj;
 }
 
 // In-place inverse of an SPD matrix: one workgroup for small n, a pair of
