@@ -852,12 +852,13 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
   if (!jk) {
     const int occ = class_occupancy(bc, kc, false, false, 0);
     const int cap = per_cu > 0 ? std::min(per_cu, occ) * c->ncu : std::min(2048, occ * c->ncu);
-    const double nbt = (nb + kTileB - 1) / kTileB;
+    const int nbt = (nb + kTileB - 1) / kTileB;
     // Ket chunks as wide as still leave kEnergyUnitsPerWave units per resident
     // wavefront: large (and sparse) systems claim up to 64 kets at once and
     // skip the screened ones inside the unit; small ones keep one ket per unit.
     int kch = c->pair_chunk > 0 ? c->pair_chunk : 1;
-    while (c->pair_chunk == 0 && kch < 64 && nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap)
+    while (c->pair_chunk == 0 && kch < 64 &&
+           (double)nbt * ((nk + 2 * kch - 1) / (2 * kch)) >= kEnergyUnitsPerWave * 4.0 * cap)
       kch *= 2;
     L.kchunk = kch;
     const long long units = (long long)nbt * ((nk + kch - 1) / kch);
@@ -865,8 +866,8 @@ static ClassLaunch straight_launch(vsvb_context* c, int bc, int kc, bool jk, int
     L.units = L.overflow ? 0 : (int)units;
     // several counter segments once claims are chunked (large systems: many
     // short claims); one counter, heaviest tiles first, otherwise
-    L.seg_tiles = kch > 1 ? kSegTiles : (int)nbt;
-    L.strips = ((int)nbt + L.seg_tiles - 1) / L.seg_tiles;
+    L.seg_tiles = kch > 1 ? kSegTiles : nbt;
+    L.strips = (nbt + L.seg_tiles - 1) / L.seg_tiles;
     L.blocks = std::min((L.units + 3) / 4, cap);
     return L;
   }
