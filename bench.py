@@ -274,7 +274,8 @@ def main():
             "batch_tuner": tuner_summary(tuning),
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64-valu",
+            "bound_note": "FP64 vector ALU: the per-quartet recursion is not a dense block product; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
             "kernel": f"{kname} ({CLASS_NAMES[q]}, J/K mode)",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,

@@ -67,6 +67,10 @@ typedef struct vsvb_options {
     int32_t pair_chunk;  /* pairs of the wave-uniform side per work-unit claim of the straight-line quartet
                           * kernels (ket pairs; bra pairs for transposed J/K units): 0 = chosen from the
                           * system size, else 1..64.  Results do not depend on it. */
+    int32_t grid_inverse_min; /* orbital overlap blocks of at least this many rows are inverted by the
+                               * grid-wide elimination (one pair of launches per pivot), smaller ones by one
+                               * workgroup: 0 = default (129), else >= 1 (1 = always grid-wide).  Results
+                               * do not depend on it beyond rounding. */
 } vsvb_options;
 
 typedef struct vsvb_context vsvb_context;

@@ -590,56 +590,68 @@ static double *eri_table(const Mol *m) {
         }
     off[npair_sh] = rows;
     double *H = (double *)calloc(rows * n2, sizeof(double));
+    /* Each shell quartet block is evaluated once, for ket pair <= bra pair, and
+     * added to both rows: (ab|cd) to the row of (ab) with ket (cd), and, by
+     * (ab|cd) = (cd|ab), to the row of (cd) with ket (ab).  Rows receive adds
+     * from several threads, so every add is atomic. */
 #pragma omp parallel
     {
         double *blk = (double *)malloc(sizeof(double) * 1296);
-        double *kt = (double *)malloc(sizeof(double) * 36 * (n2 > 0 ? n2 : 1));
 #pragma omp for schedule(dynamic, 1)
         for (int bp = 0; bp < npair_sh; ++bp) {
-            const Shell *A = &m->sh[pm[bp]], *B = &m->sh[pn[bp]];
             if (!ntouch[pm[bp]] || !ntouch[pn[bp]]) continue; /* no orbital uses this pair in either slot */
-            int nab = NCART(A->l) * NCART(B->l);
-            double *Hrow = H + off[bp] * n2;
-            for (int kp = 0; kp < npair_sh; ++kp) {
-                int L = pm[kp], S = pn[kp];
-                if (!ntouch[L] || !ntouch[S]) continue;
-                const Shell *C = &m->sh[L], *D = &m->sh[S];
+            for (int kp = 0; kp <= bp; ++kp) {
+                if (!ntouch[pm[kp]] || !ntouch[pn[kp]]) continue;
+                const Shell *A = &m->sh[pm[bp]], *B = &m->sh[pn[bp]];
+                const Shell *C = &m->sh[pm[kp]], *D = &m->sh[pn[kp]];
                 orc_eri_block(A->A, A->l, A->nprim, A->exps, A->coef, B->A, B->l, B->nprim, B->exps, B->coef,
                               C->A, C->l, C->nprim, C->exps, C->coef, D->A, D->l, D->nprim, D->exps, D->coef, blk);
-                int nC = NCART(C->l), nD = NCART(D->l), ncd = nC * nD;
-                for (int ij = 0; ij < nab; ++ij) {
-                    const double *b = blk + ij * ncd;
-                    double *h = Hrow + (size_t)ij * n2;
-                    for (int x = 0; x < ntouch[L]; ++x) {
-                        int r = touch[(size_t)L * n + x];
-                        const double *cr = cs + (size_t)r * nbf;
-                        for (int y = 0; y < ntouch[S]; ++y) {
-                            int s2 = touch[(size_t)S * n + y];
-                            const double *cq = cs + (size_t)s2 * nbf;
-                            double acc = 0.0;
-                            for (int k = 0; k < nC; ++k)
-                                for (int l = 0; l < nD; ++l) acc += b[k * nD + l] * cr[C->ao + k] * cq[D->ao + l];
-                            h[(size_t)r * n + s2] += acc;
-                        }
-                    }
-                    if (L != S)
-                        for (int x = 0; x < ntouch[S]; ++x) {
-                            int r = touch[(size_t)S * n + x];
+                int nA = NCART(A->l), nB = NCART(B->l), nC = NCART(C->l), nD = NCART(D->l);
+                int nab = nA * nB, ncd = nC * nD;
+                for (int side = 0; side < (kp < bp ? 2 : 1); ++side) {
+                    /* side 0: row pair (A,B), ket shells (C,D); side 1: row pair (C,D), ket (A,B) */
+                    const int row_pair = side ? kp : bp, nrow = side ? ncd : nab;
+                    const int L = side ? pm[bp] : pm[kp], S = side ? pn[bp] : pn[kp];
+                    const Shell *X = side ? A : C, *Y = side ? B : D;
+                    const int nX = side ? nA : nC, nY = side ? nB : nD;
+                    double *Hrow = H + off[row_pair] * n2;
+                    for (int ij = 0; ij < nrow; ++ij) {
+                        double *h = Hrow + (size_t)ij * n2;
+                        /* element (row ij, ket x y) of the block */
+#define BLK(x_, y_) (side ? blk[((x_) * nY + (y_)) * ncd + ij] : blk[ij * ncd + (x_) * nY + (y_)])
+                        for (int x = 0; x < ntouch[L]; ++x) {
+                            int r = touch[(size_t)L * n + x];
                             const double *cr = cs + (size_t)r * nbf;
-                            for (int y = 0; y < ntouch[L]; ++y) {
-                                int s2 = touch[(size_t)L * n + y];
+                            for (int y = 0; y < ntouch[S]; ++y) {
+                                int s2 = touch[(size_t)S * n + y];
                                 const double *cq = cs + (size_t)s2 * nbf;
                                 double acc = 0.0;
-                                for (int k = 0; k < nC; ++k)
-                                    for (int l = 0; l < nD; ++l) acc += b[k * nD + l] * cr[D->ao + l] * cq[C->ao + k];
+                                for (int k = 0; k < nX; ++k)
+                                    for (int l = 0; l < nY; ++l) acc += BLK(k, l) * cr[X->ao + k] * cq[Y->ao + l];
+#pragma omp atomic
                                 h[(size_t)r * n + s2] += acc;
                             }
                         }
+                        if (L != S)
+                            for (int x = 0; x < ntouch[S]; ++x) {
+                                int r = touch[(size_t)S * n + x];
+                                const double *cr = cs + (size_t)r * nbf;
+                                for (int y = 0; y < ntouch[L]; ++y) {
+                                    int s2 = touch[(size_t)L * n + y];
+                                    const double *cq = cs + (size_t)s2 * nbf;
+                                    double acc = 0.0;
+                                    for (int k = 0; k < nX; ++k)
+                                        for (int l = 0; l < nY; ++l) acc += BLK(k, l) * cr[Y->ao + l] * cq[X->ao + k];
+#pragma omp atomic
+                                    h[(size_t)r * n + s2] += acc;
+                                }
+                            }
+#undef BLK
+                    }
                 }
             }
         }
         free(blk);
-        free(kt);
     }
     double *G = (double *)calloc(n2 * n2 > 0 ? n2 * n2 : 1, sizeof(double));
 #pragma omp parallel for schedule(dynamic)

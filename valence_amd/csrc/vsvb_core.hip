@@ -612,10 +612,11 @@ __global__ void k_gj_update(double* __restrict__ a, int n, int k, const double* 
   *aij = (j == k ? 0.0 : *aij) - colk[i] * akj;
 }
 
-// In-place inverse of an SPD matrix: one workgroup for small n, a pair of
-// launches per pivot for large n.  colk: n doubles of scratch.
-static hipError_t launch_spd_inverse(hipStream_t st, double* a, int n, int* status, double* colk) {
-  if (n <= kSpdOneBlock) {
+// In-place inverse of an SPD matrix: one workgroup for n < grid_min (by
+// default up to kSpdOneBlock rows), a pair of launches per pivot from grid_min
+// rows up.  colk: n doubles of scratch.
+static hipError_t launch_spd_inverse(hipStream_t st, double* a, int n, int* status, double* colk, int grid_min) {
+  if (n < grid_min) {
     hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), n * sizeof(double), st, a, n, status);
     return hipGetLastError();
   }
@@ -956,6 +957,9 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->tau = (opt && opt->quartet_tau > 0) ? opt->quartet_tau : 1e-17;
   c->pair_chunk = opt ? opt->pair_chunk : 0;
   if (c->pair_chunk < 0 || c->pair_chunk > 64) return fail(VSVB_ERR_ARG, "pair_chunk must be 0 (automatic) or 1..64");
+  const int gim = opt ? opt->grid_inverse_min : 0;
+  if (gim < 0) return fail(VSVB_ERR_ARG, "grid_inverse_min must be 0 (default) or at least 1");
+  c->grid_inverse_min = gim > 0 ? gim : kSpdOneBlock + 1;
   HIP_OK(hipSetDevice(c->device));
   if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&c->ev0));
@@ -1311,7 +1315,7 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
     double* P = c->d_P.p + (size_t)s * ns * ns;
     if (m == 0) continue;
     hipLaunchKernelGGL(k_copy_block, dim3((m * m + 255) / 256), dim3(256), 0, st, c->d_Sorb.p, ns, off, m, P);
-    HIP_OK(launch_spd_inverse(st, P, m, c->d_status.p, c->d_colk.p));
+    HIP_OK(launch_spd_inverse(st, P, m, c->d_status.p, c->d_colk.p, c->grid_inverse_min));
     // E1 += w tr(h_s P_s): closed shell counts both spins (w = 2)
     hipLaunchKernelGGL(k_trace_prod, dim3(1), dim3(256), 0, st, c->d_Horb.p, ns, off, P, m, nspin == 1 ? 2.0 : 1.0,
                        c->d_scal.p + 0);
@@ -1387,19 +1391,28 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, kacc, nwfn * nspin * nn, 0.0);
   }
-  HIP_OK(hipEventRecord(c->ev0, st));
+  // Timing events: under stream capture a plain record is only a dependency,
+  // so they are recorded as event nodes of the graph (hipEventRecordExternal),
+  // and every replay then stamps them again.
+  const unsigned evflags = c->capturing ? hipEventRecordExternal : 0u;
+  HIP_OK(hipEventRecordWithFlags(c->ev0, st, evflags));
   size_t poff = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
       const int q = quartet_class_index(bc, kc);
       const ClassLaunch& cl = jk ? c->ljk[q] : c->le[q];
+      const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+      // a class with pairs on both sides always has work units: a launch
+      // configuration without any would drop its quartets silently
+      if (nb > 0 && nk > 0 && cl.units <= 0)
+        return fail(VSVB_ERR_UNSUPPORTED, "quartet class " + std::to_string(q) + " has no launchable work units");
       const QuartetArgs a = class_args(c, bc, kc, cl, jk, nwfn, nspin, dens, jacc, kacc, c->d_partial.p + poff);
-      if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[q], st));
+      if (c->profile) HIP_OK(hipEventRecordWithFlags(c->cls_ev[q], st, evflags));
       HIP_OK(launch_class(c, bc, kc, cl, a, jk));
       poff += c->partial_per_class[q];
     }
-  if (c->profile) HIP_OK(hipEventRecord(c->cls_ev[kNQuartetClass], st));
-  HIP_OK(hipEventRecord(c->ev1, st));
+  if (c->profile) HIP_OK(hipEventRecordWithFlags(c->cls_ev[kNQuartetClass], st, evflags));
+  HIP_OK(hipEventRecordWithFlags(c->ev1, st, evflags));
   if (!jk) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, c->d_partial.p, (long)poff, 1, e2_dev);
     HIP_OK(hipGetLastError());
@@ -1892,8 +1905,14 @@ static int tune_batch(vsvb_batch* b) {
     for (int i : g.members) {
       vsvb_context* c = b->ctx[i];
       if (c->npair > 0) continue;
-      for (int q = 0; q < kNQuartetClass; ++q)
-        if (b->tuning[k][q][7] > 0.0) c->ljk[q] = policy_launch(c, q, best[k][q]);
+      for (int q = 0; q < kNQuartetClass; ++q) {
+        if (b->tuning[k][q][7] <= 0.0) continue;
+        // the representative's best launch may not fit a larger molecule of the
+        // bucket (too many work units for 32-bit indices): that group keeps its
+        // own default launch, which its context was created with
+        const ClassLaunch cl = policy_launch(c, q, best[k][q]);
+        if (!cl.overflow && (cl.units > 0 || c->ljk[q].units <= 0)) c->ljk[q] = cl;
+      }
     }
   }
   return VSVB_OK;
@@ -2289,6 +2308,14 @@ int vsvb_set_profiling(vsvb_context* c, int32_t on) {
 
 int vsvb_batch_set_profiling(vsvb_batch* b, int32_t on) {
   if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  // a captured graph holds the per-class event nodes of the switch it was
+  // captured with: release it, the next run is direct and captures again
+  if (b->exec && (on != 0) != b->ctx[0]->profile) {
+    HIP_OK(hipSetDevice(b->ctx[0]->device));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
   for (auto* c : b->ctx) {
     const int rc = vsvb_set_profiling(c, on);
     if (rc) return rc;

@@ -123,6 +123,7 @@ struct vsvb_context {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (device entry)
   double tau = 1e-17;
   int pair_chunk = 0;  // straight-line units: uniform-side pairs per claim (0: from the system size)
+  int grid_inverse_min = 129;  // overlap blocks from this many rows up are inverted grid-wide
   // sizes
   int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
   int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;

@@ -40,7 +40,8 @@ class CSystem(C.Structure):
 
 
 class COptions(C.Structure):
-    _fields_ = [("quartet_tau", C.c_double), ("device", C.c_int32), ("pair_chunk", C.c_int32)]
+    _fields_ = [("quartet_tau", C.c_double), ("device", C.c_int32), ("pair_chunk", C.c_int32),
+                ("grid_inverse_min", C.c_int32)]
 
 
 EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_energy_parts", "vsvb_energy_grad",
@@ -146,11 +147,13 @@ class _Packed:
 class Context:
     """One molecule/basis/orbital layout resident on the GPU (vsvb_context)."""
 
-    def __init__(self, s: VsvbSystem, quartet_tau: float = 0.0, device: int = 0, pair_chunk: int = 0):
+    def __init__(self, s: VsvbSystem, quartet_tau: float = 0.0, device: int = 0, pair_chunk: int = 0,
+                 grid_inverse_min: int = 0):
         L = load()
         self.sys = s
         self._packed = _Packed(s)
-        opt = COptions(quartet_tau=quartet_tau, device=device, pair_chunk=pair_chunk)
+        opt = COptions(quartet_tau=quartet_tau, device=device, pair_chunk=pair_chunk,
+                       grid_inverse_min=grid_inverse_min)
         h = C.c_void_p()
         _check(L.vsvb_context_create(C.byref(self._packed.cs), C.byref(opt), C.byref(h)), "vsvb_context_create")
         self._h = h
@@ -251,13 +254,14 @@ class Batch:
     """Many wave functions of uneven size evaluated in one graph launch (vsvb_batch)."""
 
     def __init__(self, systems: Sequence[VsvbSystem], quartet_tau: float = 0.0, device: int = 0,
-                 pair_chunk: int = 0):
+                 pair_chunk: int = 0, grid_inverse_min: int = 0):
         L = load()
         self.systems = list(systems)
         self._packed = [_Packed(s) for s in self.systems]
         arr = (C.POINTER(CSystem) * len(self._packed))(*[C.pointer(p.cs) for p in self._packed])
         self._arr = arr
-        opt = COptions(quartet_tau=quartet_tau, device=device, pair_chunk=pair_chunk)
+        opt = COptions(quartet_tau=quartet_tau, device=device, pair_chunk=pair_chunk,
+                       grid_inverse_min=grid_inverse_min)
         h = C.c_void_p()
         _check(L.vsvb_batch_create(len(self.systems), arr, C.byref(opt), C.byref(h)), "vsvb_batch_create")
         self._h = h
