@@ -112,6 +112,21 @@ int vsvb_energy_grad(vsvb_context *ctx, const double *coords, const double *coef
 int vsvb_energy_grad_device(vsvb_context *ctx, const double *coeff_device, double *energy_device,
                             double *grad_device, void *stream);
 
+/* Allocation-free energy and gradient: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  The same evaluation as vsvb_energy_grad on a context
+ * made from sys, but with no context: every device buffer is carved from the
+ * library's static 32 MiB device scratch array (one per GPU, held until the
+ * process ends); the call allocates and frees nothing, creates no stream or
+ * event, and runs on the null stream of opt->device (opt may be NULL).
+ * Systems whose buffers exceed the array (C8H18 needs about 11 MiB, C12H26
+ * with 310 AOs about 23 MiB) and spin-coupled wave functions (npair > 0) are
+ * refused with
+ * VSVB_ERR_UNSUPPORTED.  Calls on one GPU share the array: they are
+ * serialized by a lock inside the process.  energy: [1]; grad: NULL or
+ * [totlen] (host memory). */
+int vsvb_energy_grad_static(const vsvb_system *sys, const vsvb_options *opt, double *energy, double *grad);
+
 /* The NITROGEN potential-surface plug-in (src/valence_api_nitrogen.F90):
  * calcsurface (:17-27), the energy at Cartesian coordinates x[natom][3] in
  * angstrom (converted with angs2bohr's 1.889725987722, src/tools_module.F90:11),

@@ -578,7 +578,8 @@ def write_coop(path: str) -> None:
              "// Step tables of the cooperative (one wavefront per quartet) ERI kernel.",
              f"constexpr int kNCoop = {len(descs)};",
              f"constexpr int kCoopDataLen = {len(data)};",
-             "alignas(16) static const int kCoopData[kCoopDataLen] = {"]
+             "// device-resident from module load (no allocation or upload at run time)",
+             "static __device__ __attribute__((aligned(16))) int kCoopData[kCoopDataLen] = {"]
     for i in range(0, len(data), 24):
         lines.append("  " + ", ".join(str(x) for x in data[i:i + 24]) + ",")
     lines.append("};")

@@ -950,6 +950,14 @@ static ClassLaunch coop_launch(vsvb_context* c, int q, bool jk, int per_cu, int 
   return L;
 }
 
+// A device buffer of the context: allocated, or carved from the context's
+// arena (the allocation-free entry point); an arena too small is reported with
+// the size the system needs.
+template <typename T>
+static hipError_t device_buffer(vsvb_context* c, vsvb::DevBuf<T>& b, size_t count) {
+  return c->arena ? b.carve(*c->arena, count) : b.alloc(count);
+}
+
 static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_context* c) {
   if (!s) return fail(VSVB_ERR_ARG, "null system");
   if (s->natom < 1 || s->nshell_type < 1) return fail(VSVB_ERR_ARG, "empty molecule or basis");
@@ -961,11 +969,13 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   if (gim < 0) return fail(VSVB_ERR_ARG, "grid_inverse_min must be 0 (default) or at least 1");
   c->grid_inverse_min = gim > 0 ? gim : kSpdOneBlock + 1;
   HIP_OK(hipSetDevice(c->device));
-  if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_OK(hipEventCreate(&c->ev0));
-  HIP_OK(hipEventCreate(&c->ev1));
-  HIP_OK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
-  HIP_OK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  if (!c->arena) {  // the allocation-free entry creates no stream or events: it runs on the null stream
+    if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&c->ev0));
+    HIP_OK(hipEventCreate(&c->ev1));
+    HIP_OK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  }
   c->natom = s->natom;
   c->npair = s->npair;
   c->nunpd = s->nunpd;
@@ -1098,44 +1108,44 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->coeff.assign(s->coeff, s->coeff + s->totlen);
   // device allocations (all of them, here)
   const size_t nbf2 = (size_t)c->nbf * c->nbf;
-  HIP_OK(c->d_shells.alloc(c->nsh));
-  HIP_OK(c->d_pairs.alloc(c->npairs));
-  HIP_OK(c->d_ppsrc.alloc(c->npp));
-  HIP_OK(c->d_pp.alloc(c->npp));
-  HIP_OK(c->d_pexp.alloc(pexp.size()));
-  HIP_OK(c->d_pcoef.alloc(pcoef.size()));
-  HIP_OK(c->d_xyz.alloc(3 * c->natom));
-  HIP_OK(c->d_charge.alloc(c->natom));
-  HIP_OK(c->d_boys.alloc((size_t)kBoysN * kBoysK));
-  HIP_OK(c->d_angn.alloc(c->nbf));
-  HIP_OK(c->d_ao_shell.alloc(c->nbf));
-  HIP_OK(c->d_orb_start.alloc(s->norbs + 1));
-  HIP_OK(c->d_status.alloc(4));
-  HIP_OK(c->d_colk.alloc(std::max(1, c->nsp)));
-  HIP_OK(c->d_ent.alloc(c->ent.size()));
-  HIP_OK(c->d_coeff.alloc(std::max(1, s->totlen)));
-  HIP_OK(c->d_ndf_scale.alloc(std::max(1, s->ndf)));
-  HIP_OK(c->d_S.alloc(nbf2));
-  HIP_OK(c->d_H.alloc(nbf2));
-  HIP_OK(c->d_Ct.alloc((size_t)c->nsp * c->nbf));
-  HIP_OK(c->d_Cs.alloc((size_t)c->nsp * c->nbf));
-  HIP_OK(c->d_Sorb.alloc((size_t)c->nsp * c->nsp));
-  HIP_OK(c->d_Horb.alloc((size_t)c->nsp * c->nsp));
-  HIP_OK(c->d_P.alloc(2 * (size_t)c->nsp * c->nsp));
-  HIP_OK(c->d_R.alloc(2 * nbf2));
-  HIP_OK(c->d_tmp.alloc(std::max(nbf2, (size_t)c->nsp * c->nbf)));
-  HIP_OK(c->d_tmp2.alloc(std::max(nbf2, (size_t)c->nsp * c->nbf)));
-  HIP_OK(c->d_J.alloc(nbf2));
-  HIP_OK(c->d_K.alloc(2 * nbf2));
-  HIP_OK(c->d_F.alloc(2 * nbf2));
-  HIP_OK(c->d_gradC.alloc((size_t)c->nsp * c->nbf));
-  HIP_OK(c->d_Jsym.alloc(nbf2));
-  HIP_OK(c->d_Rfull.alloc(nbf2));
-  HIP_OK(c->d_X.alloc((size_t)c->nsp * c->nbf));
-  HIP_OK(c->d_Z.alloc((size_t)c->nsp * c->nbf));
-  HIP_OK(c->d_grad.alloc(std::max(1, s->totlen)));
-  HIP_OK(c->d_scal.alloc(16));
-  HIP_OK(c->d_counters.alloc(4 * kNQuartetClass));
+  HIP_OK(device_buffer(c, c->d_shells, c->nsh));
+  HIP_OK(device_buffer(c, c->d_pairs, c->npairs));
+  HIP_OK(device_buffer(c, c->d_ppsrc, c->npp));
+  HIP_OK(device_buffer(c, c->d_pp, c->npp));
+  HIP_OK(device_buffer(c, c->d_pexp, pexp.size()));
+  HIP_OK(device_buffer(c, c->d_pcoef, pcoef.size()));
+  HIP_OK(device_buffer(c, c->d_xyz, 3 * c->natom));
+  HIP_OK(device_buffer(c, c->d_charge, c->natom));
+  HIP_OK(device_buffer(c, c->d_boys, (size_t)kBoysN * kBoysK));
+  HIP_OK(device_buffer(c, c->d_angn, c->nbf));
+  HIP_OK(device_buffer(c, c->d_ao_shell, c->nbf));
+  HIP_OK(device_buffer(c, c->d_orb_start, s->norbs + 1));
+  HIP_OK(device_buffer(c, c->d_status, 4));
+  HIP_OK(device_buffer(c, c->d_colk, std::max(1, c->nsp)));
+  HIP_OK(device_buffer(c, c->d_ent, c->ent.size()));
+  HIP_OK(device_buffer(c, c->d_coeff, std::max(1, s->totlen)));
+  HIP_OK(device_buffer(c, c->d_ndf_scale, std::max(1, s->ndf)));
+  HIP_OK(device_buffer(c, c->d_S, nbf2));
+  HIP_OK(device_buffer(c, c->d_H, nbf2));
+  HIP_OK(device_buffer(c, c->d_Ct, (size_t)c->nsp * c->nbf));
+  HIP_OK(device_buffer(c, c->d_Cs, (size_t)c->nsp * c->nbf));
+  HIP_OK(device_buffer(c, c->d_Sorb, (size_t)c->nsp * c->nsp));
+  HIP_OK(device_buffer(c, c->d_Horb, (size_t)c->nsp * c->nsp));
+  HIP_OK(device_buffer(c, c->d_P, 2 * (size_t)c->nsp * c->nsp));
+  HIP_OK(device_buffer(c, c->d_R, 2 * nbf2));
+  HIP_OK(device_buffer(c, c->d_tmp, std::max(nbf2, (size_t)c->nsp * c->nbf)));
+  HIP_OK(device_buffer(c, c->d_tmp2, std::max(nbf2, (size_t)c->nsp * c->nbf)));
+  HIP_OK(device_buffer(c, c->d_J, nbf2));
+  HIP_OK(device_buffer(c, c->d_K, 2 * nbf2));
+  HIP_OK(device_buffer(c, c->d_F, 2 * nbf2));
+  HIP_OK(device_buffer(c, c->d_gradC, (size_t)c->nsp * c->nbf));
+  HIP_OK(device_buffer(c, c->d_Jsym, nbf2));
+  HIP_OK(device_buffer(c, c->d_Rfull, nbf2));
+  HIP_OK(device_buffer(c, c->d_X, (size_t)c->nsp * c->nbf));
+  HIP_OK(device_buffer(c, c->d_Z, (size_t)c->nsp * c->nbf));
+  HIP_OK(device_buffer(c, c->d_grad, std::max(1, s->totlen)));
+  HIP_OK(device_buffer(c, c->d_scal, 16));
+  HIP_OK(device_buffer(c, c->d_counters, 4 * kNQuartetClass));
   // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -1150,7 +1160,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       c->ctr_len[q] = std::max({1, (nb + kStripB - 1) / kStripB, (nk + kStripK - 1) / kStripK});
       c->ctr_total += c->ctr_len[q];
     }
-  HIP_OK(c->d_tilectr.alloc(c->ctr_total));
+  HIP_OK(device_buffer(c, c->d_tilectr, c->ctr_total));
   HIP_OK(coop_prepare(c->device));
   c->ncu = ncu;
   c->partial_total = 0;
@@ -1170,7 +1180,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
       c->partial_per_class[q] = (size_t)c->le[q].blocks * 4;  // energy kernels write four slots per block
       c->partial_total += c->partial_per_class[q];
     }
-  HIP_OK(c->d_partial.alloc(std::max<size_t>(c->partial_total, 1) * 2));
+  HIP_OK(device_buffer(c, c->d_partial, std::max<size_t>(c->partial_total, 1) * 2));
   // uploads
   auto boys = make_boys_table();
   HIP_OK(hipMemcpy(c->d_shells.p, shells.data(), shells.size() * sizeof(Shell), hipMemcpyHostToDevice));
@@ -1395,7 +1405,7 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
   // so they are recorded as event nodes of the graph (hipEventRecordExternal),
   // and every replay then stamps them again.
   const unsigned evflags = c->capturing ? hipEventRecordExternal : 0u;
-  HIP_OK(hipEventRecordWithFlags(c->ev0, st, evflags));
+  if (c->ev0) HIP_OK(hipEventRecordWithFlags(c->ev0, st, evflags));
   size_t poff = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
@@ -1412,7 +1422,7 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       poff += c->partial_per_class[q];
     }
   if (c->profile) HIP_OK(hipEventRecordWithFlags(c->cls_ev[kNQuartetClass], st, evflags));
-  HIP_OK(hipEventRecordWithFlags(c->ev1, st, evflags));
+  if (c->ev1) HIP_OK(hipEventRecordWithFlags(c->ev1, st, evflags));
   if (!jk) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, c->d_partial.p, (long)poff, 1, e2_dev);
     HIP_OK(hipGetLastError());
@@ -1444,7 +1454,7 @@ int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
   else
     HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0.0f;
-  HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (c->ev0 && c->ev1) HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   const int nspin = c->last_nspin;
   double tot[3] = {0, 0, 0};
   for (int q = 0; q < kNQuartetClass; ++q) {
@@ -2020,6 +2030,19 @@ __global__ void k_boys_probe(const double* __restrict__ T, int n, const double* 
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point (vsvb_energy_grad_static):
+// every device buffer of its transient context is carved from this array.
+// Each GPU holds its own copy from module load until the process ends; host
+// code uses the per-device address from hipGetSymbolAddress, never the name.
+// Calls on one GPU are serialized by a host lock (they would race on the
+// array); the array keeps the last call's contents, and every buffer the
+// pipeline reads is written or cleared earlier in the same call.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static void* g_scratch_ptr[kScratchDevices] = {};
+static std::mutex g_scratch_mu[kScratchDevices];
+
 extern "C" {
 
 int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
@@ -2243,6 +2266,37 @@ int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff,
   rc = enqueue_energy_grad(c, coords, coeff, nullptr, c->d_grad.p);
   if (rc) return rc;
   return finish_energy_grad(c, energy, grad);
+}
+
+int vsvb_energy_grad_static(const vsvb_system* sys, const vsvb_options* opt, double* energy, double* grad) {
+  if (!sys || !energy) return fail(VSVB_ERR_ARG, "null argument");
+  if (sys->npair > 0)
+    return fail(VSVB_ERR_UNSUPPORTED, "the allocation-free entry covers wave functions without spin-coupled pairs "
+                                      "(npair = 0); use vsvb_context_create and vsvb_energy_grad");
+  const int dev = opt ? opt->device : 0;
+  if (dev < 0 || dev >= kScratchDevices) return fail(VSVB_ERR_ARG, "device ordinal out of range");
+  HIP_OK(hipSetDevice(dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+  if (!g_scratch_ptr[dev]) {  // once per device, with that device current
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  vsvb::Arena arena;
+  arena.base = static_cast<char*>(g_scratch_ptr[dev]);
+  arena.cap = kScratchBytes;
+  vsvb_context c;  // transient: no stream, no events, every device buffer in the arena
+  c.arena = &arena;
+  c.own_stream = false;
+  int rc = build_context(sys, opt, &c);
+  if (rc != VSVB_OK && arena.need > arena.cap)
+    return fail(VSVB_ERR_UNSUPPORTED, "the system needs more than the " + std::to_string(kScratchBytes >> 20) +
+                                          " MiB of device scratch the allocation-free entry holds (" +
+                                          std::to_string(arena.need >> 20) + " MiB requested before it ran out); "
+                                          "use vsvb_context_create and vsvb_energy_grad");
+  if (rc != VSVB_OK) return rc;
+  if ((rc = enqueue_energy_grad(&c, nullptr, nullptr, nullptr, c.d_grad.p))) return rc;
+  return finish_energy_grad(&c, energy, grad);
 }
 
 int vsvb_energy_grad_device(vsvb_context* c, const double* coeff_device, double* energy_device, double* grad_device,

@@ -299,19 +299,15 @@ int coop_occupancy(int idx, bool jk) {
   return nb > 0 ? nb : 1;
 }
 
-// The step tables on a device (uploaded once per process and device).
+// The step tables on a device: a device array initialized in the code object,
+// whose address is looked up once per device (with that device current).
 static hipError_t coop_tables(int device, const int** out) {
   std::lock_guard<std::mutex> lk(g_tab_mu);
   if (device < 0 || device >= 64) return hipErrorInvalidDevice;
   if (!g_tab[device]) {
     int* p = nullptr;
-    hipError_t e = hipMalloc((void**)&p, sizeof(kCoopData));
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(kCoopData));
     if (e != hipSuccess) return e;
-    e = hipMemcpy(p, kCoopData, sizeof(kCoopData), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(p);
-      return e;
-    }
     g_tab[device] = p;
   }
   *out = g_tab[device];

@@ -39,6 +39,14 @@ struct OrbEntry {
   int pad;
 };
 
+// A bump region of device memory that buffers are carved from instead of
+// allocated (the allocation-free entry point: the library's static scratch
+// array).  need counts every request, including those that did not fit.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0, need = 0;
+};
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -48,6 +56,18 @@ struct DevBuf {
     n = count;
     own = true;
     return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  // Take the next 256-byte-aligned slice of the arena (nothing is allocated or
+  // freed; hipErrorOutOfMemory when the arena is full).
+  hipError_t carve(Arena& a, size_t count) {
+    const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    a.need += bytes;
+    if (a.used + bytes > a.cap) return hipErrorOutOfMemory;
+    p = reinterpret_cast<T*>(a.base + a.used);
+    a.used += bytes;
+    n = count;
+    own = false;
+    return hipSuccess;
   }
   // Point at memory owned elsewhere (a batch group shares geometry data and
   // slices of group-wide density / Coulomb / exchange arrays).
@@ -124,6 +144,8 @@ struct vsvb_context {
   double tau = 1e-17;
   int pair_chunk = 0;  // straight-line units: uniform-side pairs per claim (0: from the system size)
   int grid_inverse_min = 129;  // overlap blocks from this many rows up are inverted grid-wide
+  vsvb::Arena* arena = nullptr;  // device buffers carved from this region instead of allocated (no events,
+                                 // the null stream): the allocation-free entry point
   // sizes
   int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
   int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;

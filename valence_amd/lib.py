@@ -51,7 +51,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
            "vsvb_batch_set_lanes", "vsvb_debug_gemm",
            "vsvb_spin_matrices", "vsvb_spin_opt", "vsvb_set_spin_coupling", "vsvb_surface",
-           "vsvb_batch_set_coords", "vsvb_surface_batch"]
+           "vsvb_batch_set_coords", "vsvb_surface_batch", "vsvb_energy_grad_static"]
 
 _lib = None
 
@@ -97,6 +97,7 @@ def load(path: str = LIB_PATH):
     L.vsvb_batch_tuning.argtypes = [C.c_void_p, _f64p]
     L.vsvb_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_batch_set_lanes.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_energy_grad_static.argtypes = [C.POINTER(CSystem), C.POINTER(COptions), _f64p, _f64p]
     _lib = L
     return L
 
@@ -142,6 +143,19 @@ class _Packed:
                                                 "npair", "nunpd", "ndocc", "ndf", "nspinc", "ntol_d", "ntol_i")},
                           **{k: _ptr(v, _i32p) for k, v in pk.i32.items()},
                           **{k: _ptr(v, _f64p) for k, v in pk.f64.items()})
+
+
+def energy_grad_static(s: VsvbSystem, device: int = 0, want_grad: bool = True):
+    """Energy and gradient through the allocation-free entry (vsvb_energy_grad_static):
+    no context, every device buffer in the library's static scratch array."""
+    L = load()
+    pk = _Packed(s)
+    opt = COptions(device=device)
+    e = C.c_double()
+    g = np.zeros(s.totlen) if want_grad else None
+    _check(L.vsvb_energy_grad_static(C.byref(pk.cs), C.byref(opt), C.byref(e), _ptr(g, _f64p)),
+           "vsvb_energy_grad_static")
+    return e.value, g
 
 
 class Context:
