@@ -96,7 +96,7 @@ def tuner_summary(t) -> dict:
 
 def lif_leg():
     """LiF 4x4x8 slab (128 atoms, 1536 AOs, 384 docc): one energy and one energy +
-    gradient after a warm-up of each, with per-class device times (HIP events)."""
+    gradient after a warm-up of each, with per-class device times (kernel wall-clock stamps)."""
     from valence_amd import lib as L
     from valence_amd import workloads as W
     s = W.config4()
@@ -210,9 +210,11 @@ def main():
         return
 
     # ---- roofline of the dominant kernel of the measured (batched) workload: the
-    # J/K instantiation of the class with the largest summed device time, timed by
-    # HIP events on the stream of each class launch over the timed region.  The
-    # groups overlap there (four streams), so a launch shares the GPU with
+    # J/K instantiation of the class with the largest summed device time, timed
+    # over the timed region (graph replays) by the wall-clock stamps every
+    # wavefront of the launch writes at its start and end (HIP events recorded
+    # inside a captured graph are only dependencies and would not be stamped).
+    # The groups overlap there (four streams), so a launch shares the GPU with
     # others; the same kernel is also timed alone, in one more (untimed) run
     # with the groups on one stream and no graph.
     q = int(np.argmax(stats_batch[:, 0]))
@@ -287,7 +289,8 @@ def main():
             "avg_launch_ms": avg_ms,
             "launches": nlaunch,
             "flops_per_launch": flops_per_launch,
-            "note": "launches of the timed region overlap those of other groups (four streams)",
+            "note": "launch durations from the kernels' wall-clock stamps over the timed region's graph replays; "
+                    "launches overlap those of other groups (four streams)",
         },
         "roofline_isolated": {
             "kernel": f"{kname} ({CLASS_NAMES[q]}, J/K mode)",

@@ -222,7 +222,9 @@ int vsvb_batch_class_stats(vsvb_batch *batch, double *out);
  * "gradC", "scalars".  Layouts are row-major; see the implementation for shapes. */
 int vsvb_debug_buffer(vsvb_context *ctx, const char *name, double *out, int64_t n);
 /* Evaluate the device Boys function F_0..F_L at n arguments T (diagnostics/tests):
- * out[i*(L+1) + m] = F_m(T[i]), 0 <= L <= 8. */
+ * out[i*(L+1) + m] = F_m(T[i]), 0 <= L <= 8, from the full Taylor table (the
+ * cooperative and one-electron kernels); L + 16 selects the compact per-L
+ * table that the straight-line quartet kernels stage in LDS. */
 int vsvb_debug_boys(int32_t L, const double *T, int32_t n, double *out);
 /* The library's dense product on the device (diagnostics/tests), row-major:
  * C[M x N] = alpha op(A) op(B) + beta C, op(A) M x K (A stored K x M when ta),

@@ -1,5 +1,8 @@
-"""The device Boys function (Taylor table + division-free recursions) against
-the integral definition F_m(T) = int_0^1 t^(2m) exp(-T t^2) dt at 50 digits."""
+"""The device Boys function (Taylor tables + division-free recursions) against
+the integral definition F_m(T) = int_0^1 t^(2m) exp(-T t^2) dt at 40 digits:
+the full table (dT = 0.05, 8 terms: cooperative and one-electron kernels) and
+the compact per-L table (dT = 0.2, 12 coefficients F_{L+k}/k! with the
+derivative giving F_{L+1}: the straight-line quartet kernels, staged in LDS)."""
 import mpmath as mp
 import numpy as np
 import pytest
@@ -15,10 +18,11 @@ def ref(m, T):
     return float(mp.quad(lambda t: t ** (2 * m) * mp.e ** (-T * t * t), [0, 1]))
 
 
+@pytest.mark.parametrize("compact", [False, True])
 @pytest.mark.parametrize("L", [0, 1, 4, 8])
-def test_device_boys(L):
+def test_device_boys(L, compact):
     from valence_amd import lib
-    F = lib.device_boys(L, TS)
+    F = lib.device_boys(L, TS, compact=compact)
     worst = 0.0
     for i, T in enumerate(TS):
         for m in range(L + 1):

@@ -4,8 +4,9 @@ References: the oracle (C restatement of the reference algorithm, pinned by the
 reference's 58 recorded energies) and its committed values under tests/golden/.
 Tolerances: energies 1e-9 Eh against the oracle run without the reference's
 orbital-quartet screens (the HIP path evaluates every quartet above a 1e-17
-AO Schwarz bound); gradients 1e-7 norm-wise relative against central finite
-differences with h = 1e-5.
+AO Schwarz bound) and with them; gradients 1e-7 norm-wise relative against
+central finite differences of the oracle energy with h = 1e-5
+(tests/golden/grad_fd.json).
 """
 import json
 import os
@@ -22,6 +23,7 @@ pytestmark = pytest.mark.gpu
 KNOWN = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
 ORACLE_KNOWN = json.load(open(os.path.join(GOLDEN, "oracle_known_answers.json")))
 CONFIGS = json.load(open(os.path.join(GOLDEN, "config_energies.json")))
+GRAD_FD = json.load(open(os.path.join(GOLDEN, "grad_fd.json")))
 
 
 @pytest.fixture(scope="module")
@@ -51,6 +53,7 @@ def test_known_answers_gpu(lib, key):
     orc = ORACLE_KNOWN[key]["oracle_energy"]
     print(f"  gpu={e:.16f} known={ref:.16f} oracle={orc:.16f} d_known={e - ref:+.3e} d_oracle={e - orc:+.3e}")
     assert abs(e - ref) < 1e-8
+    assert abs(e - orc) < 1e-9
 
 
 def test_config0_energy(lib):
@@ -73,26 +76,26 @@ def _unique_quartets(s):
     return T * (T + 1) // 2
 
 
-def _fd_grad(ctx, c, idx, h=1e-5):
-    g = np.zeros(len(idx))
-    for n, k in enumerate(idx):
-        cp = c.copy(); cp[k] += h
-        cm = c.copy(); cm[k] -= h
-        g[n] = (ctx.energy(cp) - ctx.energy(cm)) / (2 * h)
-    return g
+def oracle_fd(name):
+    """Central differences of the oracle energy (h = 1e-5, tests/golden/make_grad_fd.py):
+    weight indices and dE/dc values."""
+    d = GRAD_FD[name]["fd"]
+    idx = np.array(sorted(int(k) for k in d))
+    return idx, np.array([d[str(k)] for k in idx])
 
 
 def test_config0_gradient_fd(lib):
+    """Every weight's derivative against central differences of the oracle energy."""
     s = workloads.config0()
     with lib.Context(s) as ctx:
         e, g = ctx.energy_grad()
         e0 = ctx.energy(s.coeff)
-        idx = np.arange(s.totlen)
-        fd = _fd_grad(ctx, s.coeff.copy(), idx)
+    idx, fd = oracle_fd("config0")
+    assert len(idx) == s.totlen
     print(f"E(grad call)={e:.12f} E(energy call)={e0:.12f}")
     assert abs(e - e0) < 1e-9
-    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
-    print(f"|g|={np.linalg.norm(g):.6e} rel err vs FD={rel:.3e}")
+    rel = np.linalg.norm(g[idx] - fd) / np.linalg.norm(fd)
+    print(f"|g|={np.linalg.norm(g):.6e} rel err vs oracle FD={rel:.3e}")
     assert rel < 1e-7
 
 
@@ -118,6 +121,31 @@ def test_config1_energy(lib):
           f"quartets={st[0]:.0f} skipped={st[2]:.0f} eri_ms={st[3]:.2f}")
     assert abs(e - g["energy_unscreened"]) < 1e-9
     assert abs(e - g["energy"]) < 1e-9
+
+
+def test_grid_inverse_config1(lib):
+    """The grid-wide Gauss-Jordan inverse (one pair of launches per pivot), which
+    the default runs only for overlap blocks above 128 rows, forced on C8H18's
+    33 x 33 block: the oracle's energy to 1e-9 Eh, the one-workgroup inverse's
+    energy and gradient to rounding, and P S_orb = 1 at FP64 level."""
+    s = workloads.config1()
+    g = CONFIGS["config1"]
+    with lib.Context(s) as ctx:
+        e0, g0 = ctx.energy_grad()
+    with lib.Context(s, grid_inverse_min=1) as ctx:
+        e = ctx.energy()
+        e1, g1 = ctx.energy_grad()
+        n = s.nspatial
+        S = ctx.buffer("S_orb", n * n).reshape(n, n)
+        P = ctx.buffer("P", n * n).reshape(n, n)
+    res = np.abs(P @ S - np.eye(n)).max()
+    print(f"grid inverse: E={e:.12f} oracle={g['energy']:.12f} one-workgroup={e0:.12f} |PS-1|max={res:.2e}")
+    assert abs(e - g["energy"]) < 1e-9 and abs(e - g["energy_unscreened"]) < 1e-9
+    assert abs(e1 - e0) < 1e-10
+    assert np.linalg.norm(g1 - g0) <= 1e-10 * np.linalg.norm(g0)
+    assert res < 1e-12
+    with pytest.raises(lib.VsvbError, match="grid_inverse_min"):
+        lib.Context(s, grid_inverse_min=-1)
 
 
 def test_rotation_invariance(lib):

@@ -1,8 +1,8 @@
 """LiF rock-salt slabs on the GPU (through the C ABI).
 
 The 8-atom 2x2x2 slab of the LiF recipe is compared with the oracle's full
-energy (1e-9 Eh) and its gradient with central differences (h = 1e-5, 1e-7
-relative).  The 128-atom 4x4x8 slab (1536 AOs, 384 docc) is beyond the oracle's
+energy (1e-9 Eh) and its gradient with central differences of the oracle
+energy (h = 1e-5, 1e-7 relative).  The 128-atom 4x4x8 slab (1536 AOs, 384 docc) is beyond the oracle's
 reach, so it is checked through properties that hold at any size: invariance
 to scaling any orbital, to mixing orbitals that share an atom, and to a rigid
 translation (all 1e-9 relative), and its gradient through the scaling identity
@@ -20,6 +20,7 @@ from valence_amd import workloads
 pytestmark = pytest.mark.gpu
 
 CONFIGS = json.load(open(os.path.join(GOLDEN, "config_energies.json")))
+GRAD_FD = json.load(open(os.path.join(GOLDEN, "grad_fd.json")))
 
 
 @pytest.fixture(scope="module")
@@ -45,18 +46,55 @@ def test_lif8_energy(lib):
 
 
 def test_lif8_gradient_fd(lib):
+    """24 sampled weights against central differences of the oracle energy
+    (h = 1e-5, tests/golden/grad_fd.json)."""
     s = workloads.lif8()
-    h = 1e-5
+    d = GRAD_FD["lif8"]["fd"]
+    idx = np.array(sorted(int(k) for k in d))
+    fd = np.array([d[str(k)] for k in idx])
+    assert len(idx) >= 24
     with lib.Context(s) as ctx:
         e, g = ctx.energy_grad()
-        fd = np.zeros(s.totlen)
-        for k in range(s.totlen):
-            cp = s.coeff.copy(); cp[k] += h
-            cm = s.coeff.copy(); cm[k] -= h
-            fd[k] = (ctx.energy(cp) - ctx.energy(cm)) / (2 * h)
-    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
-    print(f"|g|={np.linalg.norm(g):.6e} rel err vs FD={rel:.3e}")
+    rel = np.linalg.norm(g[idx] - fd) / np.linalg.norm(fd)
+    print(f"|g|={np.linalg.norm(g):.6e} rel err vs oracle FD={rel:.3e}")
     assert rel < 1e-7
+
+
+def test_lif8_grid_inverse(lib):
+    """The grid-wide Gauss-Jordan inverse forced on LiF8's 24 x 24 overlap block:
+    the oracle's energy to 1e-9 Eh and the default path's gradient to rounding."""
+    s = workloads.lif8()
+    g = CONFIGS["lif8"]
+    with lib.Context(s) as ctx:
+        e0, g0 = ctx.energy_grad()
+    with lib.Context(s, grid_inverse_min=1) as ctx:
+        e = ctx.energy()
+        e1, g1 = ctx.energy_grad()
+    print(f"grid inverse: E={e:.12f} oracle={g['energy']:.12f} default={e0:.12f}")
+    assert abs(e - g["energy"]) < 1e-9 and abs(e - g["energy_unscreened"]) < 1e-9
+    assert abs(e1 - e0) < 1e-10
+    assert np.linalg.norm(g1 - g0) <= 1e-10 * np.linalg.norm(g0)
+
+
+def test_lif128_inverse_residual(lib, lif128):
+    """LiF128's 384 x 384 orbital overlap is inverted by the grid-wide elimination
+    (one pair of launches per pivot): P S_orb = 1 at FP64 level, and the same P
+    as the one-workgroup Gauss-Jordan inverse forced on the same block."""
+    s = lif128
+    n = s.nspatial
+    with lib.Context(s) as ctx:
+        ctx.energy()
+        S = ctx.buffer("S_orb", n * n).reshape(n, n)
+        P = ctx.buffer("P", n * n).reshape(n, n)
+    with lib.Context(s, grid_inverse_min=n + 1) as ctx:
+        ctx.energy()
+        P1 = ctx.buffer("P", n * n).reshape(n, n)
+    res = np.abs(P @ S - np.eye(n)).max()
+    dif = np.abs(P - P1).max() / np.abs(P1).max()
+    cond = np.linalg.cond(S)
+    print(f"n={n} cond(S_orb)={cond:.3e} |P S - 1|max={res:.3e} |P_grid - P_one|/|P|={dif:.3e}")
+    assert res < 1e-13 * max(1.0, cond)
+    assert dif < 1e-13 * max(1.0, cond)
 
 
 def _tol(e):

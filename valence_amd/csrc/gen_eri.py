@@ -276,11 +276,11 @@ def gen_hrr(la, lb, lc, ld, accname, em: Emitter):
     return out_map
 
 
-# Primitive-loop options per class (both off by default): KET_UNROLL[class] = 2
-# emits two ket primitives per iteration as interleaved independent chains;
-# BOYS_BRANCH_FREE selects boys_eval_bf, which keeps the loop body straight-line.
+# Primitive-loop option per class (off by default): KET_UNROLL[class] = 2
+# emits two ket primitives per iteration as interleaved independent chains.
+# The Boys function of a primitive quartet comes from the class's compact table
+# (boys_eval_c), which the quartet kernels stage in LDS.
 KET_UNROLL: Dict[Tuple[int, int, int, int], int] = {}
-BOYS_BRANCH_FREE = False
 
 
 def _suffix_block(lines, suf):
@@ -327,7 +327,7 @@ def gen_class(la, lb, lc, ld, uniform_bra=False):
     blk.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
     blk.emit("        const double pref = B.K * K.K * r;")
     blk.emit(f"        double Fb[{L + 1}];")
-    blk.emit(f"        " + ("boys_eval_bf" if BOYS_BRANCH_FREE else "boys_eval") + f"<{L}>(T, btab, Fb);")
+    blk.emit(f"        boys_eval_c<{L}>(T, btab, Fb);")
     for m in range(L + 1):
         blk.emit(f"        const double F{m} = pref * Fb[{m}];")
     if la + lb > 0:

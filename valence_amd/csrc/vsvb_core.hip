@@ -92,6 +92,30 @@ static std::vector<double> make_boys_table() {
     for (int m = K - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + ex) / (2 * m + 1);
     for (int m = 0; m < kBoysK; ++m) tab[(size_t)i * kBoysK + m] = (double)F[m];
   }
+  // compact tables: F_{L+k}(T_i) / k! for L = 0..kLTot, k < kBoysCK
+  const int KC = kLTot + kBoysCK;
+  tab.resize(kBoysCOff + (size_t)(kLTot + 1) * kBoysCLen);
+  for (int i = 0; i < kBoysCN; ++i) {
+    const long double T = (long double)i * (long double)kBoysCDT;
+    const int K = KC - 1;
+    long double term = 1.0L / (2 * K + 1), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+      term *= 2.0L * T / (2 * K + 2 * k + 1);
+      sum += term;
+      if (term < 1e-22L * sum) break;
+    }
+    const long double ex = expl(-T);
+    long double F[kLTot + kBoysCK];
+    F[K] = ex * sum;
+    for (int m = K - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + ex) / (2 * m + 1);
+    for (int L = 0; L <= kLTot; ++L) {
+      long double fact = 1.0L;
+      for (int k = 0; k < kBoysCK; ++k) {
+        if (k > 0) fact *= k;
+        tab[kBoysCOff + (size_t)L * kBoysCLen + (size_t)i * kBoysCRow + k] = (double)(F[L + k] / fact);
+      }
+    }
+  }
   return tab;
 }
 
@@ -672,6 +696,35 @@ hipError_t vsvb::launch_fill(hipStream_t st, double* p, size_t n, double v) {
 
 // Clearing counters, flags and status words is done by a kernel, not a memset,
 // so that the same code can be captured into a HIP graph.
+// Per-class timing slots before a stage: even words (earliest start) ~0, odd
+// words (latest end) 0.  A kernel, not a memset: it may be captured in a graph.
+__global__ void k_stamp_reset(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (i & 1) ? 0ull : ~0ull;
+}
+
+// After a stage: per class (one workgroup each), the earliest start and the
+// latest end over its kStampSlots slot pairs, into out[2 q], out[2 q + 1].
+__global__ void __launch_bounds__(kStampSlots) k_stamp_reduce(const unsigned long long* __restrict__ slots,
+                                                              unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long lo[kStampSlots], hi[kStampSlots];
+  const unsigned long long* s = slots + (size_t)blockIdx.x * 2 * kStampSlots;
+  lo[threadIdx.x] = s[2 * threadIdx.x];
+  hi[threadIdx.x] = s[2 * threadIdx.x + 1];
+  __syncthreads();
+  for (int o = kStampSlots / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      lo[threadIdx.x] = lo[threadIdx.x] < lo[threadIdx.x + o] ? lo[threadIdx.x] : lo[threadIdx.x + o];
+      hi[threadIdx.x] = hi[threadIdx.x] > hi[threadIdx.x + o] ? hi[threadIdx.x] : hi[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = lo[0];
+    out[2 * blockIdx.x + 1] = hi[0];
+  }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -969,6 +1022,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   if (gim < 0) return fail(VSVB_ERR_ARG, "grid_inverse_min must be 0 (default) or at least 1");
   c->grid_inverse_min = gim > 0 ? gim : kSpdOneBlock + 1;
   HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, c->device));
   if (!c->arena) {  // the allocation-free entry creates no stream or events: it runs on the null stream
     if (!c->stream) HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&c->ev0));
@@ -1116,7 +1170,7 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(device_buffer(c, c->d_pcoef, pcoef.size()));
   HIP_OK(device_buffer(c, c->d_xyz, 3 * c->natom));
   HIP_OK(device_buffer(c, c->d_charge, c->natom));
-  HIP_OK(device_buffer(c, c->d_boys, (size_t)kBoysN * kBoysK));
+  HIP_OK(device_buffer(c, c->d_boys, kBoysCOff + (size_t)(kLTot + 1) * kBoysCLen));
   HIP_OK(device_buffer(c, c->d_angn, c->nbf));
   HIP_OK(device_buffer(c, c->d_ao_shell, c->nbf));
   HIP_OK(device_buffer(c, c->d_orb_start, s->norbs + 1));
@@ -1145,7 +1199,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(device_buffer(c, c->d_Z, (size_t)c->nsp * c->nbf));
   HIP_OK(device_buffer(c, c->d_grad, std::max(1, s->totlen)));
   HIP_OK(device_buffer(c, c->d_scal, 16));
-  HIP_OK(device_buffer(c, c->d_counters, 4 * kNQuartetClass));
+  HIP_OK(device_buffer(c, c->d_counters, kCounterWords));  // 4 counters, then 2 stamps, per class
+  HIP_OK(device_buffer(c, c->d_stamps, (size_t)2 * kStampSlots * kNQuartetClass));
   // quartet launch geometry (grids sized to the occupancy of each class kernel) and partial buffer
   int ncu = 0;
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -1369,6 +1424,7 @@ static QuartetArgs class_args(vsvb_context* c, int bc, int kc, const ClassLaunch
   a.jacc = jacc;
   a.kacc = kacc;
   a.counters = c->d_counters.p + 4 * q;
+  a.stamp = c->profile ? c->d_stamps.p + (size_t)q * 2 * kStampSlots : nullptr;
   a.tile_ctr = c->d_tilectr.p + c->ctr_off[q];
   a.nstrip = cl.coop ? 1 : cl.strips;
   a.seg_tiles = cl.seg_tiles;
@@ -1401,11 +1457,12 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, kacc, nwfn * nspin * nn, 0.0);
   }
-  // Timing events: under stream capture a plain record is only a dependency,
-  // so they are recorded as event nodes of the graph (hipEventRecordExternal),
-  // and every replay then stamps them again.
-  const unsigned evflags = c->capturing ? hipEventRecordExternal : 0u;
-  if (c->ev0) HIP_OK(hipEventRecordWithFlags(c->ev0, st, evflags));
+  // Stage timing: HIP events on direct runs (under stream capture a record is
+  // only a dependency, so replays take the per-class wall-clock stamps the
+  // kernels write when profiling is on).
+  const int nslot = 2 * kStampSlots * kNQuartetClass;
+  if (c->profile) hipLaunchKernelGGL(k_stamp_reset, dim3((nslot + 255) / 256), dim3(256), 0, st, c->d_stamps.p, nslot);
+  if (c->ev0 && !c->capturing) HIP_OK(hipEventRecord(c->ev0, st));
   size_t poff = 0;
   for (int bc = 0; bc < kNPairClass; ++bc)
     for (int kc = 0; kc <= bc; ++kc) {
@@ -1417,12 +1474,13 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       if (nb > 0 && nk > 0 && cl.units <= 0)
         return fail(VSVB_ERR_UNSUPPORTED, "quartet class " + std::to_string(q) + " has no launchable work units");
       const QuartetArgs a = class_args(c, bc, kc, cl, jk, nwfn, nspin, dens, jacc, kacc, c->d_partial.p + poff);
-      if (c->profile) HIP_OK(hipEventRecordWithFlags(c->cls_ev[q], st, evflags));
       HIP_OK(launch_class(c, bc, kc, cl, a, jk));
       poff += c->partial_per_class[q];
     }
-  if (c->profile) HIP_OK(hipEventRecordWithFlags(c->cls_ev[kNQuartetClass], st, evflags));
-  if (c->ev1) HIP_OK(hipEventRecordWithFlags(c->ev1, st, evflags));
+  if (c->ev1 && !c->capturing) HIP_OK(hipEventRecord(c->ev1, st));
+  if (c->profile)
+    hipLaunchKernelGGL(k_stamp_reduce, dim3(kNQuartetClass), dim3(kStampSlots), 0, st, c->d_stamps.p,
+                       c->d_counters.p + 4 * kNQuartetClass);
   if (!jk) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, c->d_partial.p, (long)poff, 1, e2_dev);
     HIP_OK(hipGetLastError());
@@ -1441,6 +1499,11 @@ static double digestion_flops(int q, int nspin, bool jk) {
   return jk ? nab * ncd * (4.0 + 8.0 * nspin) : nab * ncd * (2.0 + 4.0 * nspin) + 2.0 * nab;
 }
 
+// Counters of the last quartet stage.  hcnt: the stage's counters and stamps
+// already gathered to host memory (a batch run), else they are read from the
+// device.  Class times come from the kernels' wall-clock stamps (profiling on);
+// the stage time from the HIP events of a direct run, or, after a graph replay
+// (whose events were not recorded), from the stamps' overall span.
 int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
   if (!c->ran_quartets) {  // a batch-group member: its integrals were counted by the leader
     std::memset(c->class_stats, 0, sizeof(c->class_stats));
@@ -1448,27 +1511,40 @@ int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
     return VSVB_OK;
   }
   c->ran_quartets = false;
-  unsigned long long h[4 * kNQuartetClass];
+  unsigned long long h[kCounterWords];
   if (hcnt)
     std::memcpy(h, hcnt, sizeof(h));
   else
     HIP_OK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
-  float ms = 0.0f;
-  if (c->ev0 && c->ev1) HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  const unsigned long long* stamp = h + 4 * kNQuartetClass;
+  const double tick_ms = c->wall_khz > 0 ? 1.0 / c->wall_khz : 0.0;
   const int nspin = c->last_nspin;
   double tot[3] = {0, 0, 0};
+  unsigned long long t0 = ~0ull, t1 = 0;
   for (int q = 0; q < kNQuartetClass; ++q) {
     for (int k = 0; k < 3; ++k) tot[k] += (double)h[4 * q + k];
-    float cms = 0.0f;
-    if (c->profile && (c->last_jk ? c->ljk[q].units : c->le[q].units) > 0)
-      HIP_OK(hipEventElapsedTime(&cms, c->cls_ev[q], c->cls_ev[q + 1]));
+    const bool launched = (c->last_jk ? c->ljk[q].units : c->le[q].units) > 0;
+    double cms = 0.0;
+    if (c->profile && launched && stamp[2 * q + 1] >= stamp[2 * q]) {
+      cms = (double)(stamp[2 * q + 1] - stamp[2 * q]) * tick_ms;
+      t0 = std::min(t0, stamp[2 * q]);
+      t1 = std::max(t1, stamp[2 * q + 1]);
+    }
     c->class_stats[q][0] = cms;
     c->class_stats[q][1] = (double)h[4 * q];
     c->class_stats[q][2] = (double)h[4 * q + 1];
     c->class_stats[q][3] = (double)h[4 * q + 2];
     c->class_stats[q][4] = (double)h[4 * q + 1] * kFlopsPerPrimQuartet[q] +
                            (double)h[4 * q] * (kFlopsHrr[q] + c->last_nwfn * digestion_flops(q, nspin, c->last_jk));
-    c->class_stats[q][5] = (c->last_jk ? c->ljk[q].units : c->le[q].units) > 0 ? 1.0 : 0.0;
+    c->class_stats[q][5] = launched ? 1.0 : 0.0;
+  }
+  double ms = std::nan("");
+  if (!hcnt && c->ev0 && c->ev1) {
+    float evms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&evms, c->ev0, c->ev1));
+    ms = evms;
+  } else if (c->profile && t1 >= t0) {
+    ms = (double)(t1 - t0) * tick_ms;
   }
   c->last_counters[0] = tot[0];
   c->last_counters[1] = tot[1];
@@ -1685,17 +1761,34 @@ static void assign_lanes(vsvb_batch* b, int nl) {
   }
 }
 
-constexpr int kCnt = 4 * kNQuartetClass;  // counters per context
+constexpr int kCnt = kCounterWords;  // counters and timing stamps per context
 
 // All device work of one batch run, no host synchronization.  The groups run
 // on the lane streams, which fork from the batch stream at the start and join
 // it at the end (in a captured graph: parallel branches).
+static int batch_enqueue_groups(vsvb_batch* b);
+
 static int batch_enqueue(vsvb_batch* b) {
-  int rc;
   HIP_OK(hipEventRecord(b->ev_fork, b->stream));
   for (auto s : b->lanes) HIP_OK(hipStreamWaitEvent(s, b->ev_fork, 0));
-  // Every wave function goes through the J/K (adjoint-ready) contraction; the
-  // energy is assembled from it, and the gradient when asked for.
+  int rc = batch_enqueue_groups(b);
+  // Every lane joins the batch stream, also after a failure: under stream
+  // capture a lane that never joins would stay in capture mode.
+  for (size_t k = 0; k < b->lanes.size(); ++k) {
+    const hipError_t e1 = hipEventRecord(b->ev_join[k], b->lanes[k]);
+    const hipError_t e2 = hipStreamWaitEvent(b->stream, b->ev_join[k], 0);
+    if (!rc && (e1 != hipSuccess || e2 != hipSuccess))
+      rc = fail(VSVB_ERR_DEVICE, std::string("joining the group streams: ") +
+                                     hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+  }
+  return rc;
+}
+
+// Every group's device work on its lane stream.  Every wave function goes
+// through the J/K (adjoint-ready) contraction; the energy is assembled from it,
+// and the gradient when asked for.
+static int batch_enqueue_groups(vsvb_batch* b) {
+  int rc;
   for (int gi : b->order) {
     BatchGroup& g = b->groups[gi];
     vsvb_context* lead = b->ctx[g.members[0]];
@@ -1721,10 +1814,6 @@ static int batch_enqueue(vsvb_batch* b) {
       m->enuc = lead->enuc;
       if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
     }
-  }
-  for (size_t k = 0; k < b->lanes.size(); ++k) {
-    HIP_OK(hipEventRecord(b->ev_join[k], b->lanes[k]));
-    HIP_OK(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
   }
   return VSVB_OK;
 }
@@ -2018,11 +2107,15 @@ static void share_geometry(vsvb_context* lead, vsvb_context* m) {
 
 // Device Boys function at given arguments (diagnostics / tests).
 template <int L>
-__global__ void k_boys_probe(const double* __restrict__ T, int n, const double* __restrict__ tab, double* __restrict__ out) {
+__global__ void k_boys_probe(const double* __restrict__ T, int n, const double* __restrict__ tab, int compact,
+                             double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double F[L + 1];
-  boys_eval<L>(T[i], tab, F);
+  if (compact)
+    boys_eval_c<L>(T[i], tab + kBoysCOff + (size_t)L * kBoysCLen, F);
+  else
+    boys_eval<L>(T[i], tab, F);
 #pragma unroll
   for (int m = 0; m <= L; ++m) out[(size_t)i * (L + 1) + m] = F[m];
 }
@@ -2046,6 +2139,8 @@ static std::mutex g_scratch_mu[kScratchDevices];
 extern "C" {
 
 int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
+  const int compact = L >= 16 ? 1 : 0;
+  if (compact) L -= 16;
   if (L < 0 || L > kLTot || n < 1 || !T || !out) return fail(VSVB_ERR_ARG, "invalid Boys probe arguments");
   auto tab = make_boys_table();
   DevBuf<double> dt, dT, dout;
@@ -2056,7 +2151,7 @@ int vsvb_debug_boys(int32_t L, const double* T, int32_t n, double* out) {
   HIP_OK(hipMemcpy(dT.p, T, n * sizeof(double), hipMemcpyHostToDevice));
   const dim3 g((n + 63) / 64), b(64);
   switch (L) {
-#define BP(LL) case LL: hipLaunchKernelGGL(k_boys_probe<LL>, g, b, 0, 0, dT.p, n, dt.p, dout.p); break;
+#define BP(LL) case LL: hipLaunchKernelGGL(k_boys_probe<LL>, g, b, 0, 0, dT.p, n, dt.p, compact, dout.p); break;
     BP(0) BP(1) BP(2) BP(3) BP(4) BP(5) BP(6) BP(7) BP(8)
 #undef BP
   }
@@ -2230,8 +2325,6 @@ void vsvb_context_destroy(vsvb_context* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
-  for (int q = 0; q <= kNQuartetClass; ++q)
-    if (c->cls_ev[q]) (void)hipEventDestroy(c->cls_ev[q]);
   hipStream_t st = c->own_stream ? c->stream : nullptr;
   delete c;
   if (st) (void)hipStreamDestroy(st);
@@ -2354,8 +2447,6 @@ int vsvb_debug_buffer(vsvb_context* c, const char* name, double* out, int64_t n)
 int vsvb_set_profiling(vsvb_context* c, int32_t on) {
   if (!c) return fail(VSVB_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
-  if (on && !c->cls_ev[0])
-    for (int q = 0; q <= kNQuartetClass; ++q) HIP_OK(hipEventCreate(&c->cls_ev[q]));
   c->profile = on != 0;
   return VSVB_OK;
 }

@@ -72,6 +72,7 @@ __global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, con
 #pragma unroll
   for (int w = 0; w < kMaxW; ++w) esum[w] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  stamp_wave(a.stamp, false);
 
   // Units are claimed a.ntk (1..1024) at a time and tested 64 at a time: each
   // lane applies the Schwarz test to one of them, and the wavefront then walks
@@ -263,6 +264,7 @@ __global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, con
         for (int k = 0; k < 4; ++k) a.partial[((size_t)blockIdx.x * 4 + k) * a.nwfn + w] = k == 0 ? s : 0.0;
     }
   }
+  stamp_wave(a.stamp, true);
   if (a.counters) {
     unsigned long long sk = nskip;  // per lane; nq and nprimq are wave-uniform
     for (int o = 32; o > 0; o >>= 1) sk += __shfl_xor(sk, o, 64);

@@ -55,7 +55,26 @@ struct QuartetArgs {
   int nstrip;           // straight-line units: energy segments of kSegTiles bra tiles, J/K strips (bra, or ket
                         // when transposed), each with its own work counter
   int lds_k;            // J/K mode: accumulate exchange rows in LDS (nbf * (NC + ND) doubles of dynamic LDS)
+  unsigned long long* stamp;  // per-class timing (null when off): kStampSlots pairs {earliest wavefront
+                              // start, latest wavefront end} in wall-clock ticks, ~0 and 0 before the launch
 };
+
+// Per-class device timing that also holds inside a replayed HIP graph: every
+// wavefront stamps its start and end on the constant-rate wall clock, into one
+// of kStampSlots slot pairs (by wavefront index, so that the atomics of a
+// launch spread over many addresses); a reduction after the stage takes the
+// minimum start and maximum end of the class.
+constexpr int kStampSlots = 256;
+__device__ __forceinline__ void stamp_wave(unsigned long long* st, bool end) {
+  if (st && (threadIdx.x & 63) == 0) {
+    const unsigned slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStampSlots - 1);
+    const unsigned long long t = (unsigned long long)wall_clock64();
+    if (end)
+      atomicMax(st + 2 * slot + 1, t);
+    else
+      atomicMin(st + 2 * slot, t);
+  }
+}
 
 // Bra pairs of an energy unit (one per lane of a wavefront), and bra tiles of
 // an energy counter segment when the class has several.
@@ -101,6 +120,14 @@ template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int kLq = LA + LB + LC + LD;
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
+  {
+    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
+    double2* dst = reinterpret_cast<double2*>(btab);
+    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
   const int n = a.nbf;
   const size_t nn = (size_t)n * n;
   const double cfac = a.nspin == 1 ? 2.0 : 1.0;  // RT = cfac * sum_s D_s
@@ -112,6 +139,7 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
 #pragma unroll
   for (int w = 0; w < kMaxW; ++w) esum[w] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  stamp_wave(a.stamp, false);
 
   // Work units (64 bra pairs x a chunk of kchunk ket pairs) are handed to
   // wavefronts one at a time from atomic counters, with no barrier, so
@@ -183,7 +211,7 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
       double v[NAB * NCD];
       if (act) {
         eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                  K.ABz, a.boys, v);
+                                  K.ABz, btab, v);
         ++nq;
         nprimq += (unsigned long long)P.npp * K.npp;
       } else {
@@ -257,6 +285,7 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
       if (lane == 0) a.partial[((size_t)blockIdx.x * 4 + wave) * a.nwfn + w] = s;
     }
   }
+  stamp_wave(a.stamp, true);
   if (a.counters) {
     unsigned long long q = nq, pq = nprimq, sk = nskip;
     for (int o = 32; o > 0; o >>= 1) {
@@ -289,6 +318,14 @@ template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NK = NC + ND;
+  constexpr int kLq = LA + LB + LC + LD;
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
+  {
+    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
+    double2* dst = reinterpret_cast<double2*>(btab);
+    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
   __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
@@ -304,6 +341,7 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
   const int kb_len = lds ? n * NK : 0;
   for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  stamp_wave(a.stamp, false);
   // Units are (bra strip, ket pair).  Every strip has its own counter of ket
   // pairs: a workgroup starts on strip blockIdx mod nstrip and moves to the
   // next strip only when its strip is exhausted, leaving after it has found
@@ -382,7 +420,7 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
       double v[NAB * NCD];
       if (act) {
         eri_block<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                  K.ABz, a.boys, v);
+                                  K.ABz, btab, v);
         ++nq;
         nprimq += (unsigned long long)P.npp * K.npp;
       } else {
@@ -515,6 +553,7 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
     }
   }
   if (racc) flush_j(jst);
+  stamp_wave(a.stamp, true);
   if (a.counters) {
     unsigned long long q = nq, pq = nprimq, sk = nskip;
     for (int o = 32; o > 0; o >>= 1) {
@@ -544,6 +583,14 @@ template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NR = NA + NB;
+  constexpr int kLq = LA + LB + LC + LD;
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
+  {
+    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
+    double2* dst = reinterpret_cast<double2*>(btab);
+    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
   __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
@@ -559,6 +606,7 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
   const int kb_len = lds ? n * NR : 0;
   for (int e = threadIdx.x; e < kb_len; e += blockDim.x) kbuf[e] = 0.0;
   unsigned long long nq = 0, nprimq = 0, nskip = 0;
+  stamp_wave(a.stamp, false);
   // One counter of bra pairs per ket strip, as in the normal units: a
   // workgroup stays on its ket strip until it is exhausted, and for one
   // density a lane keeps the Coulomb update of its ket block in registers.
@@ -631,7 +679,7 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
       double v[NAB * NCD];
       if (act) {
         eri_block_ub<LA, LB, LC, LD>(a.pp + P.pp0, P.npp, a.pp + K.pp0, K.npp, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy,
-                                     K.ABz, a.boys, v);
+                                     K.ABz, btab, v);
         ++nq;
         nprimq += (unsigned long long)P.npp * K.npp;
       } else {
@@ -763,6 +811,7 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
     }
   }
   if (racc) flush_j(jst);
+  stamp_wave(a.stamp, true);
   if (a.counters) {
     unsigned long long q = nq, pq = nprimq, sk = nskip;
     for (int o = 32; o > 0; o >>= 1) {
@@ -807,8 +856,8 @@ __global__ void __launch_bounds__(256) schwarz_kernel(Pair* pairs, const PrimPai
   for (int k = 0; k < NAB; ++k) dsum[k] = 0.0;
   for (int b = lane; b < P.npp; b += 64) {
     double v[NAB * NAB];
-    eri_block<LA, LB, LA, LB>(pp + P.pp0 + b, 1, pp + P.pp0, P.npp, P.ABx, P.ABy, P.ABz, P.ABx, P.ABy, P.ABz, boys,
-                              v);
+    eri_block<LA, LB, LA, LB>(pp + P.pp0 + b, 1, pp + P.pp0, P.npp, P.ABx, P.ABy, P.ABz, P.ABx, P.ABy, P.ABz,
+                              boys + kBoysCOff + (size_t)(2 * (LA + LB)) * kBoysCLen, v);
 #pragma unroll
     for (int k = 0; k < NAB; ++k) dsum[k] += v[k * NAB + k];
   }

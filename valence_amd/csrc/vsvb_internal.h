@@ -17,6 +17,9 @@ namespace vsvb {
 
 // Largest exchange LDS block of a J/K workgroup (nbf x ket columns doubles).
 constexpr size_t kMaxLdsK = 60 * 1024;
+// Device counter words of a quartet stage: per class {evaluated, primitive
+// quartets, skipped, spare}, then per class {earliest start, latest end} stamps.
+constexpr int kCounterWords = 6 * kNQuartetClass;
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
@@ -144,6 +147,7 @@ struct vsvb_context {
   double tau = 1e-17;
   int pair_chunk = 0;  // straight-line units: uniform-side pairs per claim (0: from the system size)
   int grid_inverse_min = 129;  // overlap blocks from this many rows up are inverted grid-wide
+  int wall_khz = 0;            // device wall-clock rate (kHz) of the kernels' timing stamps
   vsvb::Arena* arena = nullptr;  // device buffers carved from this region instead of allocated (no events,
                                  // the null stream): the allocation-free entry point
   // sizes
@@ -165,6 +169,7 @@ struct vsvb_context {
   DevBuf<PPSrc> d_ppsrc;
   DevBuf<PrimPair> d_pp;
   DevBuf<double> d_pexp, d_pcoef, d_xyz, d_charge, d_boys, d_angn;
+  DevBuf<unsigned long long> d_stamps;  // per-class wall-clock slots of the last stage (profiling)
   DevBuf<int> d_ao_shell, d_orb_start, d_status;
   DevBuf<double> d_colk;  // pivot column of the large-matrix inverse
   DevBuf<OrbEntry> d_ent;
@@ -190,7 +195,6 @@ struct vsvb_context {
   int last_nwfn = 1, last_nspin = 1;  // densities and spin blocks of the last quartet launch
   bool ran_quartets = false;  // this context launched the quartet stage in the current call
   bool capturing = false;     // enqueueing into a HIP graph capture: no host uploads
-  hipEvent_t cls_ev[vsvb::kNQuartetClass + 1] = {};
   double class_stats[vsvb::kNQuartetClass][6] = {};
   // spin-coupled wave functions (npair > 0): determinant lists and work arrays
   SpinCoupled sc;

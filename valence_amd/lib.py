@@ -112,11 +112,14 @@ def _check(rc: int, what: str):
         raise VsvbError(f"{what} failed ({rc}): {msg}")
 
 
-def device_boys(L: int, T) -> np.ndarray:
-    """F_0..F_L at the given arguments, evaluated by the device routine the ERI kernels use."""
+def device_boys(L: int, T, compact: bool = False) -> np.ndarray:
+    """F_0..F_L at the given arguments, evaluated on the device from the full table
+    (cooperative and one-electron kernels) or, compact=True, from the per-L table
+    the straight-line quartet kernels stage in LDS."""
     T = np.ascontiguousarray(T, np.float64).ravel()
     out = np.zeros((len(T), L + 1))
-    _check(load().vsvb_debug_boys(L, _ptr(T, _f64p), len(T), _ptr(out, _f64p)), "vsvb_debug_boys")
+    _check(load().vsvb_debug_boys(L + (16 if compact else 0), _ptr(T, _f64p), len(T), _ptr(out, _f64p)),
+           "vsvb_debug_boys")
     return out
 
 
