@@ -131,10 +131,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="profiling aid: enqueue every batch run directly instead of replaying a HIP graph")
     ap.add_argument("--class-stats", default="", help="write per-class statistics (JSON) to this path")
-    ap.add_argument("--leg", choices=["both", "plain", "batch", "copy", "lif"], default="both",
-                    help="profiling aid: run one leg only (no JSON line unless both); 'copy' runs a 1 GiB "
-                         "device copy (torch) used to calibrate the FETCH_SIZE/WRITE_SIZE counters; 'lif' "
-                         "times one energy and one energy + gradient of the 128-atom LiF slab")
+    ap.add_argument("--leg", choices=["both", "plain", "grad", "batch", "copy", "lif"], default="both",
+                    help="profiling aid: run one leg only (no JSON line unless both); 'grad' runs C8H18 energy + "
+                         "gradient calls (the J/K kernels of one density, as in the batch's displaced "
+                         "geometries); 'copy' runs a 1 GiB device copy (torch) used to calibrate the "
+                         "FETCH_SIZE/WRITE_SIZE counters; 'lif' times one energy and one energy + gradient of "
+                         "the 128-atom LiF slab")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU; the path shards by "
@@ -161,6 +163,22 @@ def main():
         return
 
     plain = W.config1()
+    if args.leg == "grad":
+        with L.Context(plain) as gctx:
+            gctx.set_profiling(True)
+            for _ in range(args.warmup):
+                gctx.energy_grad(plain.coeff, plain.coords)
+            stats = np.zeros((21, 6))
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                gctx.energy_grad(plain.coeff, plain.coords)
+                stats += gctx.class_stats()
+            t = time.perf_counter() - t0
+        print(json.dumps({"grad_leg": "C8H18 energy + gradient", "calls": args.steps,
+                          "ms_per_call": 1e3 * t / max(args.steps, 1),
+                          "class_ms": {CLASS_NAMES[q]: stats[q, 0] / max(args.steps, 1) for q in range(21)},
+                          "class_flops": {CLASS_NAMES[q]: stats[q, 4] / max(args.steps, 1) for q in range(21)}}))
+        return
     systems = W.config3()
     nplain = args.plain_steps or 4 * args.steps
 
