@@ -185,6 +185,25 @@ int vsvb_batch_set_graph(vsvb_batch *batch, int32_t on);
  * off).  n = 1 serializes the groups, for per-kernel timing without overlap.
  * No counterpart in the reference. */
 int vsvb_batch_set_lanes(vsvb_batch *batch, int32_t n);
+/* Class launches over many groups (no counterpart in the reference: the
+ * reference evaluates one wave function per call).  chunks >= 1:
+ * each stream's groups (closed- and high-spin; spin-coupled groups keep their
+ * own pass) run in `chunks` consecutive chunks of about equal estimated work,
+ * and each chunk launches every quartet class once over all its groups, with
+ * persistent workgroups that drain one group's work units after another;
+ * chunks = 0 (the default): every group launches its own classes, which
+ * overlaps best with the groups' orbital and gradient work on several streams.
+ * Results are the same up
+ * to the order of floating-point atomic additions.  An existing graph is
+ * released.  vsvb_batch_class_stats then reports, per class, the summed time
+ * and the number of these launches. */
+int vsvb_batch_set_group_launch(vsvb_batch *batch, int32_t chunks);
+/* The last run's group launches (no counterpart in the reference): *n = their
+ * number (0 without group launches); the first min(*n, cap) go to out as rows
+ * of 6 doubles: quartet class, kernel (0 straight-line, 1 straight-line
+ * transposed, 2 cooperative), groups, stream, device time in ms (from the
+ * kernels' wall-clock stamps; 0 unless profiling is on), algorithmic FLOPs. */
+int vsvb_batch_launches(vsvb_batch *batch, double *out, int32_t cap, int32_t *n);
 /* New coordinates (bohr) for every wave function of the batch, concatenated
  * [nwfn][natom_i][3], for the next runs (no counterpart in the reference: a
  * batch of displaced geometries, e.g. a finite-difference stencil).  Wave

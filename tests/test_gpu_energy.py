@@ -296,6 +296,40 @@ def test_pair_chunk_gradient_matches(lib):
             assert (st[0], st[1], st[2]) == ref[2:]
 
 
+def test_batch_group_launch_matches(lib):
+    """Class launches over many groups (persistent workgroups that drain one
+    group's work units after another, one to three chunks per stream) give the
+    results of one pass per group: a group of four rotations, displaced
+    geometries, alkanes (two ethanes and two pentanes share a pass each) and a
+    high-spin atom (two spin densities); direct runs and graph replays."""
+    systems = workloads.config3()
+    pick = [48, 49, 50, 51, 4, 5, 80, 84, 83, 90, 85]
+    members = [systems[i] for i in pick] + [read_input(os.path.join(GOLDEN, "inputs", "testing__b"))]
+    ref = lib.Batch(members)
+    ref.set_group_launch(0)
+    e0, g0 = ref.run(want_grad=True)
+    st0 = ref.class_stats()
+    ref.close()
+    for chunks in (1, 2, 3):
+        b = lib.Batch(members)
+        b.set_group_launch(chunks)
+        b.set_profiling(True)
+        for rep in range(2):                       # direct run, then a graph replay
+            e, g = b.run(want_grad=True)
+            st = b.class_stats()
+            for k in range(len(members)):
+                assert abs(e[k] - e0[k]) < 1e-10, (chunks, rep, k, e[k], e0[k])
+                assert np.linalg.norm(g[k] - g0[k]) <= 1e-10 * np.linalg.norm(g0[k])
+            # the same quartets; one launch per chunk and kernel variant, timed
+            assert np.array_equal(st[:, 1:4], st0[:, 1:4])
+            ran = st[:, 1] > 0
+            assert np.all(st[ran, 5] >= 1) and np.all(st[ran, 5] <= 4 * chunks * 2)
+            assert np.all(st[ran, 0] > 0)
+            print(f"chunks={chunks} rep={rep}: launches per class {st[:, 5].astype(int).tolist()}")
+        assert b.uses_graph(), b.graph_error()
+        b.close()
+
+
 def test_batch_many_density_group(lib):
     """A group of more than four densities (six rotations of C8H18) runs the
     launch policies tuned for many densities; every member must still match its

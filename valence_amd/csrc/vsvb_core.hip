@@ -812,6 +812,19 @@ static hipError_t launch_quartet_class(int bc, int kc, const QuartetArgs& a, boo
   return hipErrorInvalidValue;
 }
 
+static hipError_t launch_groups_class(int bc, int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                                      unsigned long long* stamp, hipStream_t st) {
+  switch (bc) {
+    case 0: return launch_groups_b0(kc, gs, ng, tr, nblocks, lds, stamp, st);
+    case 1: return launch_groups_b1(kc, gs, ng, tr, nblocks, lds, stamp, st);
+    case 2: return launch_groups_b2(kc, gs, ng, tr, nblocks, lds, stamp, st);
+    case 3: return launch_groups_b3(kc, gs, ng, tr, nblocks, lds, stamp, st);
+    case 4: return launch_groups_b4(kc, gs, ng, tr, nblocks, lds, stamp, st);
+    case 5: return launch_groups_b5(kc, gs, ng, tr, nblocks, lds, stamp, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 static int class_occupancy(int bc, int kc, bool jk, bool tr, size_t lds) {
   switch (bc) {
     case 0: return occupancy_b0(kc, jk, tr, lds);
@@ -1437,26 +1450,38 @@ static hipError_t launch_class(vsvb_context* c, int bc, int kc, const ClassLaunc
   return launch_quartet_class(bc, kc, a, jk, cl.tr, cl.blocks, cl.lds, c->stream);
 }
 
+// Before a quartet pass: the first `words` counter words (quartets, primitive
+// quartets, skips per class; all words also clear the timing results), the
+// work counters and, in J/K mode, the accumulators.
+static hipError_t quartet_pass_clear(vsvb_context* c, bool jk, int nwfn, int nspin, double* jacc, double* kacc,
+                                     int words) {
+  hipStream_t st = c->stream;
+  const size_t nn = (size_t)c->nbf * c->nbf;
+  hipError_t e;
+  if ((e = launch_zero(st, c->d_counters.p, words * sizeof(unsigned long long))) != hipSuccess) return e;
+  if ((e = launch_zero(st, c->d_tilectr.p, c->ctr_total * sizeof(int))) != hipSuccess) return e;
+  c->last_jk = jk;
+  c->last_nwfn = nwfn;
+  c->last_nspin = nspin;
+  c->ran_quartets = true;
+  c->ev_valid = false;
+  if (jk) {
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, kacc, nwfn * nspin * nn, 0.0);
+  }
+  return hipGetLastError();
+}
+
 // nspin_override = 0 turns the J/K launch into a Coulomb-only pass over nwfn
 // general symmetric densities (the exchange accumulators are not touched).
 int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const double* dens, double* jacc,
                        double* kacc, int nspin_override) {
   hipStream_t st = c->stream;
   const int nspin = nspin_override >= 0 ? nspin_override : (c->nunpd > 0 ? 2 : 1);
-  const size_t nn = (size_t)c->nbf * c->nbf;
   if (!dens) dens = c->d_R.p;
   if (!jacc) jacc = c->d_J.p;
   if (!kacc) kacc = c->d_K.p;
-  HIP_OK(launch_zero(st, c->d_counters.p, 4 * kNQuartetClass * sizeof(unsigned long long)));
-  HIP_OK(launch_zero(st, c->d_tilectr.p, c->ctr_total * sizeof(int)));
-  c->last_jk = jk;
-  c->last_nwfn = nwfn;
-  c->last_nspin = nspin;
-  c->ran_quartets = true;
-  if (jk) {
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, jacc, nwfn * nn, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, kacc, nwfn * nspin * nn, 0.0);
-  }
+  HIP_OK(quartet_pass_clear(c, jk, nwfn, nspin, jacc, kacc, 4 * kNQuartetClass));
   // Stage timing: HIP events on direct runs (under stream capture a record is
   // only a dependency, so replays take the per-class wall-clock stamps the
   // kernels write when profiling is on).
@@ -1477,7 +1502,10 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
       HIP_OK(launch_class(c, bc, kc, cl, a, jk));
       poff += c->partial_per_class[q];
     }
-  if (c->ev1 && !c->capturing) HIP_OK(hipEventRecord(c->ev1, st));
+  if (c->ev0 && c->ev1 && !c->capturing) {
+    HIP_OK(hipEventRecord(c->ev1, st));
+    c->ev_valid = true;
+  }
   if (c->profile)
     hipLaunchKernelGGL(k_stamp_reduce, dim3(kNQuartetClass), dim3(kStampSlots), 0, st, c->d_stamps.p,
                        c->d_counters.p + 4 * kNQuartetClass);
@@ -1539,7 +1567,7 @@ int vsvb::finish_counters(vsvb_context* c, const unsigned long long* hcnt) {
     c->class_stats[q][5] = launched ? 1.0 : 0.0;
   }
   double ms = std::nan("");
-  if (!hcnt && c->ev0 && c->ev1) {
+  if (!hcnt && c->ev_valid) {
     float evms = 0.0f;
     HIP_OK(hipEventElapsedTime(&evms, c->ev0, c->ev1));
     ms = evms;
@@ -1694,11 +1722,52 @@ struct BatchGroup {
 constexpr int kTuneBuckets = 2;  // groups of one wave function (and of a few), groups of more
 constexpr int kTuneCols = 8;
 constexpr int kBatchLanes = 4;  // streams the groups of a batch run on
+// Group-launch chunks per lane: 0 (one quartet pass per group) by default.  On
+// the 96-wave-function C8H18 batch (MI355X) one launch per class over a
+// stream's groups cut the serialized quartet time from 423 to 336 ms per run
+// (one stream: 471 -> 379 ms per run), but on four streams the per-group
+// passes overlap better with the groups' orbital and gradient work: 338-340 ms
+// against 348-365 ms for 1-3 chunks on 2-4 streams.
+constexpr int kDefaultChunks = 0;
+
+// One launch of a quartet class over several groups of a lane (the argument
+// records d_qargs[off, off + ng)).
+struct GroupLaunch {
+  int q = 0, lane = 0, off = 0, ng = 0, blocks = 0;
+  int coop_idx = -1;  // cooperative kernel (step-table index), or -1: straight-line kernel
+  bool tr = false;    // straight-line: transposed J/K units
+  size_t lds = 0;     // dynamic LDS: the largest of the groups' exchange staging
+  std::vector<int> groups;  // the groups whose records it takes, in record order
+  double ms = 0.0, flops = 0.0;  // last run: span of the launch (profiling on), algorithmic FLOPs of its groups
+};
+
+// Groups of one lane that run as one unit: their geometry and orbital work,
+// then one launch per class over all of them, then their gradients.
+struct LaneChunk {
+  int lane = 0;
+  std::vector<int> groups;  // in enqueue order
+  int l0 = 0, l1 = 0;       // its launches glaunch[l0, l1)
+};
 
 struct vsvb_batch {
   std::vector<vsvb_context*> ctx;
   std::vector<BatchGroup> groups;
   std::vector<int> order;                // groups in enqueue order (estimated cost, largest first)
+  std::vector<int> lane_of;              // lane of each group
+  std::vector<double> gcost;             // estimated work of each group
+  // Class launches over many groups (vsvb_batch_set_group_launch): chunks per
+  // lane (0: every group launches its own classes), the plan of the last
+  // direct run, its argument records on the device and the launches' timing.
+  int chunks = kDefaultChunks;
+  bool planned = false;
+  std::vector<LaneChunk> lchunks;
+  std::vector<GroupLaunch> glaunch;
+  std::vector<QuartetArgs> h_qargs;
+  DevBuf<QuartetArgs> d_qargs;
+  DevBuf<unsigned long long> d_lstamp;   // kStampSlots slot pairs per launch
+  DevBuf<unsigned long long> d_lspan;    // per launch: first wavefront start, last wavefront end
+  std::vector<unsigned long long> h_lspan;
+  double lstats[kNQuartetClass][2] = {};  // per class: summed launch time (ms), launches
   hipStream_t stream = nullptr;          // the run's own stream: fork, join, gather, graph launch
   std::vector<hipStream_t> lanes;        // group streams (default priority), forked from and joined to stream
   hipEvent_t ev_fork = nullptr;
@@ -1754,11 +1823,15 @@ static void assign_lanes(vsvb_batch* b, int nl) {
   for (size_t k = 0; k < ng; ++k) b->order[k] = (int)k;
   std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   std::vector<double> load(nl, 0.0);
+  b->lane_of.assign(ng, 0);
+  b->gcost = cost;
   for (int gi : b->order) {
     const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[l] += cost[gi];
+    b->lane_of[gi] = l;
     for (int i : b->groups[gi].members) b->ctx[i]->stream = b->lanes[l];
   }
+  b->planned = false;
 }
 
 constexpr int kCnt = kCounterWords;  // counters and timing stamps per context
@@ -1784,36 +1857,218 @@ static int batch_enqueue(vsvb_batch* b) {
   return rc;
 }
 
+static hipError_t launch_group_rec(vsvb_batch* b, const GroupLaunch& r, unsigned long long* stamp) {
+  const QuartetArgs* gs = b->d_qargs.p + r.off;
+  hipStream_t st = b->lanes[r.lane];
+  if (r.coop_idx >= 0) return launch_coop_groups(r.coop_idx, gs, r.ng, r.blocks, b->ctx[0]->device, stamp, st);
+  const int bc = class_bra(r.q), kc = r.q - quartet_class_index(bc, 0);
+  return launch_groups_class(bc, kc, gs, r.ng, r.tr, r.blocks, r.lds, stamp, st);
+}
+
+template <typename T>
+static hipError_t ensure_buffer(DevBuf<T>& d, size_t count) {
+  if (d.p && d.n >= count) return hipSuccess;
+  if (d.p && d.own) (void)hipFree(d.p);
+  d.p = nullptr;
+  return d.alloc(count);
+}
+
+// Plan of the class launches over many groups: every lane's closed- and
+// high-spin groups (a spin-coupled group runs its own pass) in `chunks`
+// consecutive chunks of about equal estimated work; per chunk and class one
+// launch over the chunk's groups, or one per kernel variant when the groups'
+// launch policies differ (cooperative or straight-line kernel, orientation).
+// Each group keeps its own work units, claims and buffers (its argument
+// record); the launch takes the largest grid and exchange staging among them.
+// The records go to the device here, on a direct run before any capture, and
+// stay unchanged while a captured graph refers to them.
+static int batch_plan(vsvb_batch* b) {
+  b->planned = false;
+  b->lchunks.clear();
+  b->glaunch.clear();
+  b->h_qargs.clear();
+  if (b->chunks <= 0) return VSVB_OK;
+  for (int l = 0; l < (int)b->lanes.size(); ++l) {
+    std::vector<int> gl;
+    double tot = 0.0;
+    for (int gi : b->order)
+      if (b->lane_of[gi] == l) {
+        gl.push_back(gi);
+        tot += b->gcost[gi];
+      }
+    if (gl.empty()) continue;
+    const int K = std::min<int>(b->chunks, (int)gl.size());
+    const size_t c0 = b->lchunks.size();
+    for (int k = 0; k < K; ++k) {
+      b->lchunks.emplace_back();
+      b->lchunks.back().lane = l;
+    }
+    double run = 0.0;
+    for (int gi : gl) {
+      const int k = std::min(K - 1, (int)(run * K / std::max(tot, 1e-300)));
+      b->lchunks[c0 + k].groups.push_back(gi);
+      run += b->gcost[gi];
+    }
+  }
+  for (auto& ch : b->lchunks) {
+    ch.l0 = (int)b->glaunch.size();
+    for (int q = 0; q < kNQuartetClass; ++q) {
+        const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
+        // kernel variants of this class in the chunk: key -1 - coop index (cooperative), 0 / 1 (orientation)
+        std::vector<int> keys;
+        std::vector<std::vector<QuartetArgs>> args;
+        std::vector<GroupLaunch> recs;
+        for (int gi : ch.groups) {
+          BatchGroup& g = b->groups[gi];
+          vsvb_context* lead = b->ctx[g.members[0]];
+          if (lead->npair > 0) continue;
+          const int nb = lead->pair_off[bc + 1] - lead->pair_off[bc], nk = lead->pair_off[kc + 1] - lead->pair_off[kc];
+          if (nb <= 0 || nk <= 0) continue;
+          const ClassLaunch& cl = lead->ljk[q];
+          if (cl.units <= 0)
+            return fail(VSVB_ERR_UNSUPPORTED, "quartet class " + std::to_string(q) + " has no launchable work units");
+          const int nwfn = (int)g.members.size();
+          const int nspin = lead->nunpd > 0 ? 2 : 1;
+          QuartetArgs a = class_args(lead, bc, kc, cl, true, nwfn, nspin, nwfn > 1 ? g.dens.p : lead->d_R.p,
+                                     nwfn > 1 ? g.jacc.p : lead->d_J.p, nwfn > 1 ? g.kacc.p : lead->d_K.p,
+                                     lead->d_partial.p);
+          a.stamp = nullptr;  // the launch is timed as a whole
+          const int key = cl.coop ? -1 - lead->coop_idx[q] : (cl.tr ? 1 : 0);
+          size_t v = 0;
+          while (v < keys.size() && keys[v] != key) ++v;
+          if (v == keys.size()) {
+            keys.push_back(key);
+            args.emplace_back();
+            GroupLaunch r;
+            r.q = q;
+            r.lane = ch.lane;
+            r.coop_idx = cl.coop ? lead->coop_idx[q] : -1;
+            r.tr = !cl.coop && cl.tr;
+            recs.push_back(r);
+          }
+          args[v].push_back(a);
+          recs[v].groups.push_back(gi);
+          recs[v].blocks = std::max(recs[v].blocks, cl.blocks);
+          recs[v].lds = std::max(recs[v].lds, cl.coop ? (size_t)0 : cl.lds);
+        }
+        for (size_t v = 0; v < keys.size(); ++v) {
+          recs[v].off = (int)b->h_qargs.size();
+          recs[v].ng = (int)args[v].size();
+          b->h_qargs.insert(b->h_qargs.end(), args[v].begin(), args[v].end());
+          b->glaunch.push_back(recs[v]);
+        }
+      }
+    ch.l1 = (int)b->glaunch.size();
+  }
+  const size_t nlaunch = b->glaunch.size();
+  HIP_OK(ensure_buffer(b->d_qargs, std::max<size_t>(b->h_qargs.size(), 1)));
+  HIP_OK(ensure_buffer(b->d_lstamp, std::max<size_t>(nlaunch, 1) * 2 * kStampSlots));
+  HIP_OK(ensure_buffer(b->d_lspan, std::max<size_t>(nlaunch, 1) * 2));
+  if (!b->h_qargs.empty())
+    HIP_OK(hipMemcpy(b->d_qargs.p, b->h_qargs.data(), b->h_qargs.size() * sizeof(QuartetArgs), hipMemcpyHostToDevice));
+  b->h_lspan.assign(2 * nlaunch, 0);
+  b->planned = true;
+  return VSVB_OK;
+}
+
+// Geometry and orbital work of one group, and, for a spin-coupled group, its
+// whole pass; with own = true also its own quartet pass and its gradients.
+static int group_front(vsvb_batch* b, BatchGroup& g, bool own) {
+  int rc;
+  vsvb_context* lead = b->ctx[g.members[0]];
+  for (int i : g.members)
+    for (int k = 0; k < b->ctx[i]->totlen; ++k)
+      if (b->ctx[i]->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
+  // geometry-dependent data are rebuilt every run: passing the coordinates forces it
+  if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
+  if (lead->npair > 0) {
+    if ((rc = prepare_orbital_basis(lead, nullptr, nullptr))) return rc;
+    return sc_enqueue(lead, true, lead->d_grad.p);
+  }
+  for (int i : g.members)
+    if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
+  const int nwfn = (int)g.members.size();
+  if (!own) {
+    const int nspin = lead->nunpd > 0 ? 2 : 1;
+    HIP_OK(quartet_pass_clear(lead, true, nwfn, nspin, nwfn > 1 ? g.jacc.p : lead->d_J.p,
+                              nwfn > 1 ? g.kacc.p : lead->d_K.p, kCounterWords));
+    return VSVB_OK;
+  }
+  if (nwfn > 1) {
+    if ((rc = run_quartets(lead, true, nullptr, nwfn, g.dens.p, g.jacc.p, g.kacc.p))) return rc;
+  } else if ((rc = run_quartets(lead, true, nullptr))) {
+    return rc;
+  }
+  return VSVB_OK;
+}
+
+static int group_gradients(vsvb_batch* b, BatchGroup& g) {
+  vsvb_context* lead = b->ctx[g.members[0]];
+  if (lead->npair > 0) return VSVB_OK;
+  for (int i : g.members) {
+    vsvb_context* m = b->ctx[i];
+    m->enuc = lead->enuc;
+    const int rc = gradient_impl(m, m->d_grad.p);
+    if (rc) return rc;
+  }
+  return VSVB_OK;
+}
+
 // Every group's device work on its lane stream.  Every wave function goes
 // through the J/K (adjoint-ready) contraction; the energy is assembled from it,
-// and the gradient when asked for.
+// and the gradient when asked for.  With a plan of group launches, each lane
+// chunk runs its groups' geometry and orbital work, then one launch per class
+// over the chunk, then the gradients; otherwise every group runs its own pass.
 static int batch_enqueue_groups(vsvb_batch* b) {
   int rc;
-  for (int gi : b->order) {
-    BatchGroup& g = b->groups[gi];
-    vsvb_context* lead = b->ctx[g.members[0]];
-    for (int i : g.members)
-      for (int k = 0; k < b->ctx[i]->totlen; ++k)
-        if (b->ctx[i]->ent[k].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "batched wave functions with NDFs");
-    // geometry-dependent data are rebuilt every run: passing the coordinates forces it
-    if ((rc = prepare_geometry(lead, lead->coords.data()))) return rc;
-    if (lead->npair > 0) {
-      if ((rc = prepare_orbital_basis(lead, nullptr, nullptr))) return rc;
-      if ((rc = sc_enqueue(lead, true, lead->d_grad.p))) return rc;
-      continue;
+  if (!b->planned || b->chunks <= 0) {
+    for (int gi : b->order) {
+      BatchGroup& g = b->groups[gi];
+      if ((rc = group_front(b, g, true))) return rc;
+      if ((rc = group_gradients(b, g))) return rc;
     }
-    for (int i : g.members)
-      if ((rc = prepare_orbitals(b->ctx[i], nullptr, nullptr))) return rc;
-    if (g.members.size() > 1) {
-      if ((rc = run_quartets(lead, true, nullptr, (int)g.members.size(), g.dens.p, g.jacc.p, g.kacc.p))) return rc;
-    } else if ((rc = run_quartets(lead, true, nullptr))) {
-      return rc;
+    return VSVB_OK;
+  }
+  const bool prof = b->ctx[0]->profile;
+  for (const auto& ch : b->lchunks) {
+    for (int gi : ch.groups)
+      if ((rc = group_front(b, b->groups[gi], false))) return rc;
+    hipStream_t st = b->lanes[ch.lane];
+    const int nl = ch.l1 - ch.l0;
+    unsigned long long* stamps = b->d_lstamp.p + (size_t)ch.l0 * 2 * kStampSlots;
+    if (prof && nl > 0) {
+      const int nslot = nl * 2 * kStampSlots;
+      hipLaunchKernelGGL(k_stamp_reset, dim3((nslot + 255) / 256), dim3(256), 0, st, stamps, nslot);
     }
-    for (int i : g.members) {
-      vsvb_context* m = b->ctx[i];
-      m->enuc = lead->enuc;
-      if ((rc = gradient_impl(m, m->d_grad.p))) return rc;
-    }
+    for (int r = ch.l0; r < ch.l1; ++r)
+      HIP_OK(launch_group_rec(b, b->glaunch[r], prof ? b->d_lstamp.p + (size_t)r * 2 * kStampSlots : nullptr));
+    if (prof && nl > 0)
+      hipLaunchKernelGGL(k_stamp_reduce, dim3(nl), dim3(kStampSlots), 0, st, stamps, b->d_lspan.p + 2 * (size_t)ch.l0);
+    HIP_OK(hipGetLastError());
+    for (int gi : ch.groups)
+      if ((rc = group_gradients(b, b->groups[gi]))) return rc;
+  }
+  return VSVB_OK;
+}
+
+// Per-class launch times of the last run's group launches (profiling on).
+static int batch_launch_stats(vsvb_batch* b) {
+  std::memset(b->lstats, 0, sizeof(b->lstats));
+  if (!b->planned || b->chunks <= 0) return VSVB_OK;
+  const bool prof = b->ctx[0]->profile;
+  if (prof && !b->glaunch.empty())
+    HIP_OK(hipMemcpy(b->h_lspan.data(), b->d_lspan.p, b->h_lspan.size() * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  const double tick_ms = b->ctx[0]->wall_khz > 0 ? 1.0 / b->ctx[0]->wall_khz : 0.0;
+  for (size_t r = 0; r < b->glaunch.size(); ++r) {
+    GroupLaunch& L = b->glaunch[r];
+    const int q = L.q;
+    b->lstats[q][1] += 1.0;
+    const unsigned long long t0 = b->h_lspan[2 * r], t1 = b->h_lspan[2 * r + 1];
+    L.ms = (prof && t1 >= t0) ? (double)(t1 - t0) * tick_ms : 0.0;
+    b->lstats[q][0] += L.ms;
+    L.flops = 0.0;  // the groups' leads counted their quartets (finish_counters ran before)
+    for (int gi : L.groups) L.flops += b->ctx[b->groups[gi].members[0]]->class_stats[q][4];
   }
   return VSVB_OK;
 }
@@ -2468,13 +2723,19 @@ int vsvb_batch_set_profiling(vsvb_batch* b, int32_t on) {
   return VSVB_OK;
 }
 
-/* Sum of the per-class statistics of every wave function of the last run. */
+/* Sum of the per-class statistics of every wave function of the last run; with
+   group launches the time and launch columns are those of the launches. */
 int vsvb_batch_class_stats(vsvb_batch* b, double* out) {
   if (!b || !out) return fail(VSVB_ERR_ARG, "null argument");
   std::memset(out, 0, sizeof(double) * kNQuartetClass * 6);
   for (auto* c : b->ctx)
     for (int q = 0; q < kNQuartetClass; ++q)
       for (int k = 0; k < 6; ++k) out[q * 6 + k] += c->class_stats[q][k];
+  if (b->planned && b->chunks > 0)
+    for (int q = 0; q < kNQuartetClass; ++q) {
+      out[q * 6] = b->lstats[q][0];
+      out[q * 6 + 5] = b->lstats[q][1];
+    }
   return VSVB_OK;
 }
 
@@ -2576,7 +2837,11 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
     if ((rc = tune_batch(b))) return rc;
   }
   ++b->runs;
-  if (b->exec) return batch_replay(b, want_grad, energies, grads);
+  if (b->exec) {
+    if ((rc = batch_replay(b, want_grad, energies, grads))) return rc;
+    return batch_launch_stats(b);
+  }
+  if ((rc = batch_plan(b))) return rc;
   if ((rc = batch_enqueue(b))) return rc;
   for (size_t i = 0; i < b->ctx.size(); ++i) {
     vsvb_context* c = b->ctx[i];
@@ -2584,6 +2849,7 @@ int vsvb_batch_run(vsvb_batch* b, int32_t want_grad, double* energies, double* c
     double* g = (want_grad && grads) ? grads[i] : nullptr;
     if ((rc = c->npair > 0 ? sc_finish(c, e, nullptr, g) : finish_energy_grad(c, e, g))) return rc;
   }
+  if ((rc = batch_launch_stats(b))) return rc;
   // capturing runs nothing: the graph is built now, after the first (direct) run,
   // and replayed from the next run on; on failure runs stay direct
   if (!b->graph_failed && !b->graph_off && !b->graph_hold) (void)batch_capture(b);
@@ -2660,6 +2926,40 @@ int vsvb_batch_set_lanes(vsvb_batch* b, int32_t n) {
     b->exec = nullptr;
   }
   assign_lanes(b, nl);
+  return VSVB_OK;
+}
+
+/* The last run's group launches, rows of 6: class, kernel (0 straight-line, 1
+   straight-line transposed, 2 cooperative), groups, stream, ms, FLOPs. */
+int vsvb_batch_launches(vsvb_batch* b, double* out, int32_t cap, int32_t* n) {
+  if (!b || !n || (cap > 0 && !out)) return fail(VSVB_ERR_ARG, "null argument");
+  const bool on = b->planned && b->chunks > 0;
+  *n = on ? (int32_t)b->glaunch.size() : 0;
+  for (int32_t r = 0; r < std::min(*n, cap); ++r) {
+    const GroupLaunch& L = b->glaunch[r];
+    double* o = out + 6 * (size_t)r;
+    o[0] = L.q;
+    o[1] = L.coop_idx >= 0 ? 2 : (L.tr ? 1 : 0);
+    o[2] = L.ng;
+    o[3] = L.lane;
+    o[4] = L.ms;
+    o[5] = L.flops;
+  }
+  return VSVB_OK;
+}
+
+int vsvb_batch_set_group_launch(vsvb_batch* b, int32_t chunks) {
+  if (!b) return fail(VSVB_ERR_ARG, "null batch");
+  if (chunks < 0 || chunks > 64) return fail(VSVB_ERR_ARG, "group-launch chunks per lane must be 0..64");
+  HIP_OK(hipSetDevice(b->ctx[0]->device));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (auto s : b->lanes) HIP_OK(hipStreamSynchronize(s));
+  if (b->exec) {
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  b->chunks = chunks;
+  b->planned = false;
   return VSVB_OK;
 }
 

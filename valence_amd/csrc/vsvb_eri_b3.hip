@@ -13,6 +13,17 @@ hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, bool tr, int
   }
 }
 
+hipError_t launch_groups_b3(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st) {
+  switch (kc) {
+    case 0: return launch_q_groups<2, 0, 0, 0>(gs, ng, tr, nblocks, lds, stamp, st);
+    case 1: return launch_q_groups<2, 0, 1, 0>(gs, ng, tr, nblocks, lds, stamp, st);
+    case 2: return launch_q_groups<2, 0, 1, 1>(gs, ng, tr, nblocks, lds, stamp, st);
+    case 3: return launch_q_groups<2, 0, 2, 0>(gs, ng, tr, nblocks, lds, stamp, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 int occupancy_b3(int kc, bool jk, bool tr, size_t lds) {
   switch (kc) {
     case 0: return occupancy_q<2, 0, 0, 0>(jk, tr, lds);

@@ -48,8 +48,8 @@ __device__ __forceinline__ void boys_any(int L, double T, const double* __restri
 __device__ __forceinline__ double pick3(int d, double x, double y, double z) { return d == 0 ? x : (d == 1 ? y : z); }
 
 template <bool kJK>
-__global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, const int* __restrict__ tab) {
-  extern __shared__ double S[];
+__device__ __forceinline__ void coop_body(const QuartetArgs& a, const CoopDesc& d, const int* __restrict__ tab,
+                                          double* S) {
   const int lane = threadIdx.x;
   const int NA = ncart(d.la), NB = ncart(d.lb), NC = ncart(d.lc), ND = ncart(d.ld);
   const int NAB = NA * NB, NCD = NC * ND;
@@ -276,6 +276,27 @@ __global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, con
   }
 }
 
+template <bool kJK>
+__global__ void __launch_bounds__(64) coop_kernel(QuartetArgs a, CoopDesc d, const int* __restrict__ tab) {
+  extern __shared__ double S[];
+  coop_body<kJK>(a, d, tab, S);
+}
+
+// J/K mode over several groups (molecules) of a batch in one launch: the
+// persistent wavefronts drain group 0's units, then group 1's, and so on (the
+// same scheme as the straight-line kernels' multi-group launch).
+__global__ void __launch_bounds__(64) coop_groups_kernel(const QuartetArgs* __restrict__ gs, int ng, CoopDesc d,
+                                                         const int* __restrict__ tab, unsigned long long* stamp) {
+  extern __shared__ double S[];
+  stamp_wave(stamp, false);
+  for (int g = 0; g < ng; ++g) {
+    const QuartetArgs a = gs[g];
+    if (a.ntiles > 0) coop_body<true>(a, d, tab, S);
+    __syncthreads();  // the group's last unit is done with the LDS
+  }
+  stamp_wave(stamp, true);
+}
+
 std::mutex g_tab_mu;
 int* g_tab[64] = {};
 
@@ -331,6 +352,17 @@ hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int 
     hipLaunchKernelGGL(coop_kernel<true>, dim3(nblocks), dim3(64), lds, st, a, kCoopDesc[idx], t);
   else
     hipLaunchKernelGGL(coop_kernel<false>, dim3(nblocks), dim3(64), lds, st, a, kCoopDesc[idx], t);
+  return hipGetLastError();
+}
+
+hipError_t launch_coop_groups(int idx, const QuartetArgs* gs, int ng, int nblocks, int device,
+                              unsigned long long* stamp, hipStream_t st) {
+  if (nblocks <= 0 || ng <= 0) return hipSuccess;
+  const int* t = nullptr;
+  hipError_t e = coop_tables(device, &t);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(coop_groups_kernel, dim3(nblocks), dim3(64), coop_lds_bytes(idx), st, gs, ng, kCoopDesc[idx], t,
+                     stamp);
   return hipGetLastError();
 }
 

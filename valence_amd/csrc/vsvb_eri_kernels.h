@@ -116,18 +116,20 @@ struct DBlocks {
   }
 };
 
+// The compact Boys table of total angular momentum kLq (F_{kLq+k}(T_i)/k!),
+// copied into the workgroup's LDS by all its threads before any use.
+template <int kLq>
+__device__ __forceinline__ void stage_boys(const double* __restrict__ boys, double* btab) {
+  const double2* src = reinterpret_cast<const double2*>(boys + kBoysCOff + (size_t)kLq * kBoysCLen);
+  double2* dst = reinterpret_cast<double2*>(btab);
+  for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
+  __syncthreads();
+}
+
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
+__device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a, const double* __restrict__ btab) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
-  constexpr int kLq = LA + LB + LC + LD;
-  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
-  {
-    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
-    double2* dst = reinterpret_cast<double2*>(btab);
-    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
-    __syncthreads();
-  }
   const int n = a.nbf;
   const size_t nn = (size_t)n * n;
   const double cfac = a.nspin == 1 ? 2.0 : 1.0;  // RT = cfac * sum_s D_s
@@ -315,17 +317,9 @@ __device__ __forceinline__ void quartet_energy_body(const QuartetArgs& a) {
 constexpr int kStripB = 4 * kTileB;
 
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
+__device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a, const double* __restrict__ btab) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NK = NC + ND;
-  constexpr int kLq = LA + LB + LC + LD;
-  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
-  {
-    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
-    double2* dst = reinterpret_cast<double2*>(btab);
-    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
-    __syncthreads();
-  }
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
   __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
@@ -580,17 +574,9 @@ __device__ __forceinline__ void quartet_jk_body(const QuartetArgs& a) {
 constexpr int kStripK = 256;
 
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
+__device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a, const double* __restrict__ btab) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NR = NA + NB;
-  constexpr int kLq = LA + LB + LC + LD;
-  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
-  {
-    const double2* src = reinterpret_cast<const double2*>(a.boys + kBoysCOff + (size_t)kLq * kBoysCLen);
-    double2* dst = reinterpret_cast<double2*>(btab);
-    for (int e = threadIdx.x; e < kBoysCLen / 2; e += blockDim.x) dst[e] = src[e];
-    __syncthreads();
-  }
   extern __shared__ double kbuf[];
   __shared__ int s_unit;
   __shared__ double s_qmax[4];  // per-wavefront largest Schwarz bound of the strip
@@ -829,15 +815,46 @@ __device__ __forceinline__ void quartet_jkt_body(const QuartetArgs& a) {
 
 template <int LA, int LB, int LC, int LD>
 __global__ void __launch_bounds__(256) quartet_jkt_kernel(QuartetArgs a) {
-  quartet_jkt_body<LA, LB, LC, LD>(a);
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
+  stage_boys<LA + LB + LC + LD>(a.boys, btab);
+  quartet_jkt_body<LA, LB, LC, LD>(a, btab);
 }
 
 template <int LA, int LB, int LC, int LD, bool kJK>
 __global__ void __launch_bounds__(256) quartet_kernel(QuartetArgs a) {
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];  // this class's compact Boys table
+  stage_boys<LA + LB + LC + LD>(a.boys, btab);
   if constexpr (kJK)
-    quartet_jk_body<LA, LB, LC, LD>(a);
+    quartet_jk_body<LA, LB, LC, LD>(a, btab);
   else
-    quartet_energy_body<LA, LB, LC, LD>(a);
+    quartet_energy_body<LA, LB, LC, LD>(a, btab);
+}
+
+// One J/K launch of a class over several groups (molecules) of a batch, with
+// persistent workgroups: they drain group 0's work units, then group 1's, and
+// so on.  Each group's units come from its own counters, so a workgroup that
+// finds one group exhausted starts on the next while others finish the
+// previous one: no launch ends with a tail, and one launch replaces one per
+// group.  Every group's arguments hold its own buffers; the Boys table values
+// are the same in every context, so the first group's is staged.  stamp: the
+// launch's wall-clock slots (profiling) or null.
+template <int LA, int LB, int LC, int LD, bool kTr>
+__global__ void __launch_bounds__(256) quartet_jk_groups_kernel(const QuartetArgs* __restrict__ gs, int ng,
+                                                               unsigned long long* stamp) {
+  __shared__ __attribute__((aligned(16))) double btab[kBoysCLen];
+  stage_boys<LA + LB + LC + LD>(gs[0].boys, btab);
+  stamp_wave(stamp, false);
+  for (int g = 0; g < ng; ++g) {
+    const QuartetArgs a = gs[g];
+    if (a.ntiles > 0) {
+      if constexpr (kTr)
+        quartet_jkt_body<LA, LB, LC, LD>(a, btab);
+      else
+        quartet_jk_body<LA, LB, LC, LD>(a, btab);
+    }
+    __syncthreads();  // the group's LDS blocks are flushed before the next group's zeroing
+  }
+  stamp_wave(stamp, true);
 }
 
 // Schwarz bound of every pair of one class, Q = sqrt(max_ab |(ab|ab)|): one
@@ -919,13 +936,43 @@ static hipError_t launch_q(const QuartetArgs& a, bool jk, bool tr, int nblocks, 
   return hipGetLastError();
 }
 
+template <int LA, int LB, int LC, int LD>
+static hipError_t launch_q_groups(const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds_bytes,
+                                  unsigned long long* stamp, hipStream_t st) {
+  if (nblocks <= 0 || ng <= 0) return hipSuccess;
+  if constexpr (LA != LC || LB != LD) {
+    if (tr) {
+      hipLaunchKernelGGL((quartet_jk_groups_kernel<LA, LB, LC, LD, true>), dim3(nblocks), dim3(256), lds_bytes, st, gs,
+                         ng, stamp);
+      return hipGetLastError();
+    }
+  } else {
+    if (tr) return hipErrorInvalidValue;  // no transposed kernel for a class paired with itself
+  }
+  hipLaunchKernelGGL((quartet_jk_groups_kernel<LA, LB, LC, LD, false>), dim3(nblocks), dim3(256), lds_bytes, st, gs, ng,
+                     stamp);
+  return hipGetLastError();
+}
+
 // Per-bra-class launchers (one translation unit each).
 hipError_t launch_quartet_b0(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b0(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 hipError_t launch_quartet_b1(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b1(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 hipError_t launch_quartet_b2(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b2(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 hipError_t launch_quartet_b3(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b3(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 hipError_t launch_quartet_b4(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b4(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 hipError_t launch_quartet_b5(int kc, const QuartetArgs& a, bool jk, bool tr, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_groups_b5(int kc, const QuartetArgs* gs, int ng, bool tr, int nblocks, size_t lds,
+                            unsigned long long* stamp, hipStream_t st);
 int occupancy_b0(int kc, bool jk, bool tr, size_t lds);
 size_t scratch_b0(int kc, bool jk, bool tr);
 int occupancy_b1(int kc, bool jk, bool tr, size_t lds);

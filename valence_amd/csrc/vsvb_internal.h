@@ -143,6 +143,7 @@ struct vsvb_context {
   hipStream_t stream = nullptr;
   bool own_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ev_valid = false;  // ev0 / ev1 bracket the last quartet pass (a direct run_quartets)
   hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (device entry)
   double tau = 1e-17;
   int pair_chunk = 0;  // straight-line units: uniform-side pairs per claim (0: from the system size)
@@ -222,6 +223,10 @@ int coop_occupancy(int idx, bool jk);
 bool coop_default(int idx);
 hipError_t coop_prepare(int device);              // step tables resident on the device
 hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int device, hipStream_t st);
+// J/K mode over ng groups' arguments (device array gs) in one launch; stamp: the
+// launch's wall-clock slots or null
+hipError_t launch_coop_groups(int idx, const QuartetArgs* gs, int ng, int nblocks, int device,
+                              unsigned long long* stamp, hipStream_t st);
 
 // spin-coupled path (vsvb_sc.hip)
 int sc_setup(vsvb_context* c, const vsvb_system* s);

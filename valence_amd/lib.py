@@ -49,7 +49,7 @@ EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_e
            "vsvb_last_error", "vsvb_last_counters", "vsvb_set_profiling", "vsvb_class_stats",
            "vsvb_batch_set_profiling", "vsvb_batch_class_stats", "vsvb_debug_buffer", "vsvb_debug_boys",
            "vsvb_version", "vsvb_batch_uses_graph", "vsvb_batch_tuning", "vsvb_batch_set_graph",
-           "vsvb_batch_set_lanes", "vsvb_debug_gemm",
+           "vsvb_batch_set_lanes", "vsvb_batch_set_group_launch", "vsvb_batch_launches", "vsvb_debug_gemm",
            "vsvb_spin_matrices", "vsvb_spin_opt", "vsvb_set_spin_coupling", "vsvb_surface",
            "vsvb_batch_set_coords", "vsvb_surface_batch", "vsvb_energy_grad_static"]
 
@@ -97,6 +97,8 @@ def load(path: str = LIB_PATH):
     L.vsvb_batch_tuning.argtypes = [C.c_void_p, _f64p]
     L.vsvb_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
     L.vsvb_batch_set_lanes.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_batch_set_group_launch.argtypes = [C.c_void_p, C.c_int32]
+    L.vsvb_batch_launches.argtypes = [C.c_void_p, _f64p, C.c_int32, C.POINTER(C.c_int32)]
     L.vsvb_energy_grad_static.argtypes = [C.POINTER(CSystem), C.POINTER(COptions), _f64p, _f64p]
     _lib = L
     return L
@@ -301,6 +303,20 @@ class Batch:
     def set_lanes(self, n: int):
         """Number of streams the groups run on (1 serializes them)."""
         _check(load().vsvb_batch_set_lanes(self._h, int(n)), "vsvb_batch_set_lanes")
+
+    def set_group_launch(self, chunks: int):
+        """Class launches over the groups of each stream, in this many chunks per
+        stream (0: one pass per group)."""
+        _check(load().vsvb_batch_set_group_launch(self._h, int(chunks)), "vsvb_batch_set_group_launch")
+
+    def launches(self) -> np.ndarray:
+        """The last run's group launches: rows of (class, kernel: 0 straight-line,
+        1 transposed, 2 cooperative, groups, stream, ms, FLOPs)."""
+        n = C.c_int32(0)
+        _check(load().vsvb_batch_launches(self._h, None, 0, C.byref(n)), "vsvb_batch_launches")
+        out = np.zeros((max(n.value, 1), 6))
+        _check(load().vsvb_batch_launches(self._h, _ptr(out, _f64p), n.value, C.byref(n)), "vsvb_batch_launches")
+        return out[:n.value]
 
     def set_coords(self, coords_bohr):
         """New coordinates for every wave function (list of [natom, 3] arrays, bohr)."""
