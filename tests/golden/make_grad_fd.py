@@ -11,9 +11,17 @@ Cases and weights:
 * config2 (H12 spin-coupled) and lif8 (LiF 2x2x2): 24 weights drawn with
   numpy.random.default_rng(4200 + case index);
 * the reference's spin-coupled inputs testing/be-sc, testing/h2o-vdz-sc1,
-  examples/lih.SCval: every weight, at the seeded perturbation the GPU tests
-  use (their recorded weights are optimized, |dE/dc| ~ 1e-5):
+  examples/lih.SCval and its NDF inputs testing/be+ndf, testing/h+ndf,
+  testing/cu+, examples/cu+.3d10 (orbitals expanded in NDFs, NDF weights
+  included): every weight, at the seeded perturbation the GPU tests use
+  (their recorded weights are optimized, |dE/dc| ~ 1e-5):
   c + 0.05 * N(0,1)[default_rng(7)] * max(|c|, 0.1).
+For the NDF inputs also a directional derivative along a seeded unit
+direction u (numpy.random.default_rng(3300 + case index), over all weights)
+by the five-point stencil with h = 1e-3, whose rounding error (~1e-15 |E| / h)
+stays below 1e-7 of the derivative also where the gradient is small next to
+|E| (examples/cu+.3d10: |g| ~ 0.2 at |E| ~ 1600 Eh, where the h = 1e-5
+differences carry ~3e-7 of rounding noise).
 Run: python tests/golden/make_grad_fd.py [nthreads]
 """
 import json
@@ -33,6 +41,7 @@ from valence_amd.system import read_input  # noqa: E402
 OUT = os.path.join(ROOT, "tests", "golden", "grad_fd.json")
 H = 1e-5
 RECORDED_SC = ["testing/be-sc", "testing/h2o-vdz-sc1", "examples/lih.SCval"]
+RECORDED_NDF = ["testing/be+ndf", "testing/h+ndf", "testing/cu+", "examples/cu+.3d10"]
 
 
 def perturbed(s):
@@ -63,7 +72,7 @@ def case_weights(name, s):
 def main(nthreads):
     data = json.load(open(OUT)) if os.path.exists(OUT) else {}
     data["h"] = H
-    for name in ["config0", "config2", "lif8"] + RECORDED_SC:
+    for name in ["config0", "config2", "lif8"] + RECORDED_SC + RECORDED_NDF:
         s = case_system(name)
         ent = data.setdefault(name, {"title": s.title, "fd": {}})
         for k in case_weights(name, s):
@@ -77,6 +86,23 @@ def main(nthreads):
             ent["fd"][str(k)] = (ep - em) / (2 * H)
             json.dump(data, open(OUT, "w"), indent=1, sort_keys=True)
         print(f"{name}: {len(ent['fd'])} components", flush=True)
+    for ci, name in enumerate(RECORDED_NDF):
+        ent = data[name]
+        if "dir5" in ent:
+            continue
+        s = case_system(name)
+        u = np.random.default_rng(3300 + ci).standard_normal(s.totlen)
+        u /= np.linalg.norm(u)
+        hd = 1e-3
+        ev = {}
+        for m in (-2, -1, 1, 2):
+            t = s.copy()
+            t.coeff = s.coeff + m * hd * u
+            ev[m] = oracle.energy(t, nthreads=nthreads).energy
+        ent["dir5"] = {"seed": 3300 + ci, "h": hd,
+                       "value": (-ev[2] + 8 * ev[1] - 8 * ev[-1] + ev[-2]) / (12 * hd)}
+        json.dump(data, open(OUT, "w"), indent=1, sort_keys=True)
+        print(f"{name}: directional derivative {ent['dir5']['value']:.12e}", flush=True)
 
 
 if __name__ == "__main__":

@@ -17,6 +17,7 @@ with their optimization groups and method as given.
 import json
 import os
 
+import numpy as np
 import pytest
 
 from conftest import GOLDEN
@@ -35,7 +36,7 @@ def _case(name):
 
 
 @pytest.mark.parametrize("name", ["h2o-vdz-dem", "li_opt"])
-def test_dem_matches_reference(name):
+def test_dem_matches_reference(name, tmp_path):
     from valence_amd.optimize import dem_minimize
     s, rec = _case(name)
     assert s.ptbnmax > 0
@@ -45,6 +46,28 @@ def test_dem_matches_reference(name):
           f"energy calls={r.iterations}")
     assert abs(r.history[0] - rec["guess_energy"]) < 1e-8
     assert abs(r.energy - rec["total_energy"]) < 1e-6
+    # the 'orbitals' file continues the run: read back into the deck, its orbitals
+    # give the energy of the returned weights (to the file's eight decimals); the
+    # reference's restore after two failed trials leaves that above the tracked
+    # total (h2o-vdz-dem: 5.8e-4 Eh), the file reports the tracked total
+    from valence_amd import lib
+    from valence_amd.optimize import save_orbitals
+    from valence_amd.system import parse_input, write_input
+    save_orbitals(s, r, str(tmp_path))
+    text = open(tmp_path / "orbitals").read()
+    assert " converged to " in text
+    s2 = parse_input(write_input(s, orbitals=False) + text)
+    with lib.Context(s) as ctx:
+        e_direct = ctx.energy(r.coeff)
+        e_round = ctx.energy(np.round(r.coeff, 8))
+    with lib.Context(s2) as ctx:
+        e2 = ctx.energy()
+    print(f"  at the final weights: E={e_direct:.12f}; rounded to 8 decimals: E={e_round:.12f}; "
+          f"from the orbitals file: E={e2:.12f}; |c| in [{np.min(np.abs(r.coeff[r.coeff != 0])):.3e}, "
+          f"{np.max(np.abs(r.coeff)):.3e}]; max |file - c| {np.max(np.abs(s2.coeff - r.coeff)):.2e}")
+    assert abs(e_direct - r.energy_at_weights) < 1e-9
+    assert abs(e2 - r.energy_at_weights) < 1e-6
+    assert f"{r.energy:32.16f}" in text
 
 
 @pytest.mark.parametrize("name", ["h2o-vdz", "h2o-vdz-dem", "li_opt"])

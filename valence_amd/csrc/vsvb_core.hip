@@ -752,7 +752,8 @@ __global__ void k_sym_combine(const double* __restrict__ X, double a, const doub
   M[t] = v;
 }
 
-// grad[k] = angn[mu_k] * G[orb_k][mu_k] for AO entries (NDF-carrying inputs are rejected on the host).
+// grad[k] = angn[mu_k] * G[orb_k][mu_k] for the AO entries of occupied orbitals,
+// 0 for the rest (the weights involving NDFs follow from k_ndf_grad).
 __global__ void k_gather_grad(const OrbEntry* __restrict__ ent, int totlen, int nsp, const double* __restrict__ angn,
                               const double* __restrict__ G, int nbf, double* __restrict__ grad) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -761,9 +762,76 @@ __global__ void k_gather_grad(const OrbEntry* __restrict__ ent, int totlen, int 
   grad[k] = (e.kind == 0 && e.orb < nsp) ? angn[e.tgt] * G[(size_t)e.orb * nbf + e.tgt] : 0.0;
 }
 
+// Gradient of the weights that involve NDFs (ndf2obs, src/valence.F90:2191-2247,
+// with the NDF normalization of :144).  Orbital p's effective AO weights are
+//   C_p = sum_{AO entries k} angn c_k e_mu(k) + sum_{entries k -> NDF f} c_k s_f d_f,
+//   d_f = sum_{weights l of f} angn c_l e_mu(l),   s_f = (d_f^T S d_f)^(-1/2),
+// so that, with G_p = dE/dC_p,
+//   entry k -> f of orbital p:   dE/dc_k = s_f (d_f . G_p)
+//   weight l of NDF f:           dE/dc_l = s_f angn (g_f - s_f^2 (d_f . g_f) S d_f)[mu(l)],
+//   g_f = sum over every entry k -> f (any occupied orbital p) of c_k G_p.
+// One workgroup (256 threads) per NDF; dv, gv: [ndf][nbf] scratch.
+__global__ void __launch_bounds__(256) k_ndf_grad(const OrbEntry* __restrict__ ent, const int* __restrict__ orb_start,
+                                                  const double* __restrict__ coeff, const double* __restrict__ angn,
+                                                  const double* __restrict__ ndf_scale, const double* __restrict__ S,
+                                                  const double* __restrict__ G, int nsp, int nbf, int first_ndf,
+                                                  double* __restrict__ dv, double* __restrict__ gv,
+                                                  double* __restrict__ grad) {
+  __shared__ double red[256];
+  const int f = blockIdx.x, o = first_ndf + f, tid = threadIdx.x;
+  double* d = dv + (size_t)f * nbf;
+  double* g = gv + (size_t)f * nbf;
+  const double sf = ndf_scale[f];
+  auto block_sum = [&](double v) {
+    red[tid] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (tid < h) red[tid] += red[tid + h];
+      __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  };
+  for (int mu = tid; mu < nbf; mu += 256) d[mu] = 0.0;
+  __syncthreads();
+  if (tid == 0)
+    for (int l = orb_start[o]; l < orb_start[o + 1]; ++l) d[ent[l].tgt] += angn[ent[l].tgt] * coeff[l];
+  for (int mu = tid; mu < nbf; mu += 256) {
+    double t = 0.0;
+    for (int p = 0; p < nsp; ++p)
+      for (int k = orb_start[p]; k < orb_start[p + 1]; ++k)
+        if (ent[k].kind == 1 && ent[k].tgt == o) t += coeff[k] * G[(size_t)p * nbf + mu];
+    g[mu] = t;
+  }
+  __syncthreads();
+  double t = 0.0;
+  for (int mu = tid; mu < nbf; mu += 256) t += d[mu] * g[mu];
+  const double dg = block_sum(t);
+  for (int p = 0; p < nsp; ++p)
+    for (int k = orb_start[p]; k < orb_start[p + 1]; ++k) {
+      if (ent[k].kind != 1 || ent[k].tgt != o) continue;  // the same for every thread
+      double u = 0.0;
+      for (int mu = tid; mu < nbf; mu += 256) u += d[mu] * G[(size_t)p * nbf + mu];
+      const double v = block_sum(u);
+      if (tid == 0) grad[k] = sf * v;
+    }
+  for (int l = orb_start[o]; l < orb_start[o + 1]; ++l) {
+    const int mu = ent[l].tgt;
+    double u = 0.0;
+    for (int nu = tid; nu < nbf; nu += 256) u += S[(size_t)mu * nbf + nu] * d[nu];
+    const double sd = block_sum(u);
+    if (tid == 0) grad[l] = sf * angn[mu] * (g[mu] - sf * sf * dg * sd);
+  }
+}
+
 hipError_t vsvb::launch_gather_grad(vsvb_context* c, const double* gradC, double* grad) {
   hipLaunchKernelGGL(k_gather_grad, dim3((c->totlen + 255) / 256), dim3(256), 0, c->stream, c->d_ent.p, c->totlen,
                      c->nsp, c->d_angn.p, gradC, c->nbf, grad);
+  if (c->ndf > 0)
+    hipLaunchKernelGGL(k_ndf_grad, dim3(c->ndf), dim3(256), 0, c->stream, c->d_ent.p, c->d_orb_start.p, c->d_coeff.p,
+                       c->d_angn.p, c->d_ndf_scale.p, c->d_S.p, gradC, c->nsp, c->nbf, c->first_ndf, c->d_ndf_d.p,
+                       c->d_ndf_g.p, grad);
   return hipGetLastError();
 }
 
@@ -1192,6 +1260,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   HIP_OK(device_buffer(c, c->d_ent, c->ent.size()));
   HIP_OK(device_buffer(c, c->d_coeff, std::max(1, s->totlen)));
   HIP_OK(device_buffer(c, c->d_ndf_scale, std::max(1, s->ndf)));
+  HIP_OK(device_buffer(c, c->d_ndf_d, s->ndf > 0 ? (size_t)s->ndf * c->nbf : 1));
+  HIP_OK(device_buffer(c, c->d_ndf_g, s->ndf > 0 ? (size_t)s->ndf * c->nbf : 1));
   HIP_OK(device_buffer(c, c->d_S, nbf2));
   HIP_OK(device_buffer(c, c->d_H, nbf2));
   HIP_OK(device_buffer(c, c->d_Ct, (size_t)c->nsp * c->nbf));
@@ -1670,17 +1740,13 @@ static int gradient_impl(vsvb_context* c, double* grad_dev) {
     HIP_OK(gemm(st, m, n, n, -1.0, c->d_Z.p, n, false, c->d_S.p, n, false, 1.0, c->d_X.p, n));    // X -= Z S
     HIP_OK(gemm(st, m, n, m, 2.0, P, m, false, c->d_X.p, n, false, 1.0, c->d_gradC.p + (size_t)off * n, n));
   }
-  hipLaunchKernelGGL(k_gather_grad, dim3((c->totlen + 255) / 256), dim3(256), 0, st, c->d_ent.p, c->totlen, ns,
-                     c->d_angn.p, c->d_gradC.p, n, grad_dev);
-  HIP_OK(hipGetLastError());
+  HIP_OK(launch_gather_grad(c, c->d_gradC.p, grad_dev));
   return VSVB_OK;
 }
 
 static int enqueue_energy_grad(vsvb_context* c, const double* coords, const double* coeff_host,
                                const double* coeff_dev, double* grad_dev) {
   int rc;
-  for (int i = 0; i < c->totlen; ++i)
-    if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
   if ((rc = prepare_geometry(c, coords))) return rc;
   if ((rc = prepare_orbitals(c, coeff_host, coeff_dev))) return rc;
   if ((rc = run_quartets(c, true, nullptr))) return rc;
@@ -2604,8 +2670,6 @@ int vsvb_energy_grad(vsvb_context* c, const double* coords, const double* coeff,
   HIP_OK(hipSetDevice(c->device));
   int rc;
   if (c->npair > 0) {
-    for (int i = 0; i < c->totlen; ++i)
-      if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
     if ((rc = prepare_geometry(c, coords))) return rc;
     if ((rc = prepare_orbital_basis(c, coeff, nullptr))) return rc;
     if ((rc = sc_enqueue(c, true, c->d_grad.p))) return rc;
@@ -2651,8 +2715,6 @@ int vsvb_energy_grad_device(vsvb_context* c, const double* coeff_device, double*
                             void* stream) {
   if (!c || !coeff_device || !energy_device || !grad_device) return fail(VSVB_ERR_ARG, "null argument");
   HIP_OK(hipSetDevice(c->device));
-  for (int i = 0; i < c->totlen; ++i)
-    if (c->ent[i].kind != 0) return fail(VSVB_ERR_UNSUPPORTED, "gradient with NDF-expanded orbitals");
   hipStream_t caller = (hipStream_t)stream;
   // the caller's earlier work (that produced coeff) -> the context stream -> the caller's later work
   HIP_OK(hipEventRecord(c->ev_in, caller));

@@ -174,6 +174,7 @@ struct vsvb_context {
   DevBuf<int> d_ao_shell, d_orb_start, d_status;
   DevBuf<double> d_colk;  // pivot column of the large-matrix inverse
   DevBuf<OrbEntry> d_ent;
+  DevBuf<double> d_ndf_d, d_ndf_g;  // NDF gradient scratch: [ndf][nbf] each
   DevBuf<double> d_coeff, d_ndf_scale, d_S, d_H, d_Ct, d_Cs, d_Sorb, d_Horb, d_P, d_R, d_tmp, d_tmp2;
   DevBuf<double> d_partial, d_scal, d_J, d_K, d_F, d_gradC, d_grad, d_Jsym, d_Rfull, d_X, d_Z;
   DevBuf<unsigned long long> d_counters;

@@ -36,7 +36,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import lib
-from .system import VsvbSystem
+from .system import VsvbSystem, write_orbitals
 
 KCAL_PER_HARTREE = 627.509469  # the reference's tokcal (src/valence.F90:2759)
 
@@ -49,6 +49,20 @@ class OptResult:
     iterations: int
     gradient_norm: float            # over the optimized weights, at the end
     history: List[float] = field(default_factory=list)
+    # the energy of the returned weights; differs from ``energy`` after a DEM
+    # whose last trials failed (see dem_minimize)
+    energy_at_weights: float = float("nan")
+
+
+def save_orbitals(s: VsvbSystem, r: "OptResult", directory: str, mode: str = "done") -> None:
+    """The optimized orbitals as the reference leaves them (xm_output after
+    minimize_energy, src/valence.F90:2882): 'orbitals' (and 'nelecwfn' with
+    several spin couplings) in ``directory``; mode 'done' adds the tolerance
+    10^-ntol_e_max kcal/mol the run converged to."""
+    t = s.copy()
+    t.coeff = np.asarray(r.coeff, dtype=np.float64).copy()
+    t.coeff_sc = np.asarray(r.coeff_sc, dtype=np.float64).copy()
+    write_orbitals(t, r.energy, directory, mode, 10.0 ** -int(s.ntol_e_max))
 
 
 def free_weights(s: VsvbSystem) -> np.ndarray:
@@ -108,14 +122,20 @@ def minimize_energy(s: VsvbSystem, device: int = 0, max_passes: int = 50, gtol: 
         if s.nspinc > 1:
             _, csc_out = ctx.spin_opt(c)
         return OptResult(energy=hist[-1], coeff=c, coeff_sc=csc_out, iterations=iters, gradient_norm=gnorm,
-                         history=hist)
+                         history=hist, energy_at_weights=hist[-1])
 
 
 def dem_minimize(s: VsvbSystem, device: int = 0) -> OptResult:
     """The reference's DEM optimizer (demgs_opt inside minimize_energy's loops,
     src/valence.F90:351-521, 2744-2885) on the library's energies.  Returns the
     energy the reference tracks and prints as the total (after a failed trial
-    the weight is restored and the previous energy kept, :494-500)."""
+    the weight is restored and the previous energy kept, :494-500).  As in the
+    reference, that restore resets one weight to its value before the trials,
+    while the normalizations after the two trials have rescaled the orbital's
+    other weights: the orbital changes shape and the kept energy is no longer
+    that of the weights (h2o-vdz-dem: the returned weights lie 5.8e-4 Eh above
+    the tracked total).  ``energy_at_weights`` is the energy of the returned
+    weights, which is what the 'orbitals' file of either program reproduces."""
     if s.nspinc > 1:
         raise ValueError("DEM with several spin couplings also runs spin_opt; not restated here")
     c = np.array(s.coeff, dtype=np.float64)
@@ -194,5 +214,6 @@ def dem_minimize(s: VsvbSystem, device: int = 0) -> OptResult:
                         hist.append(energy)
                         setcnv = abs(energy - eprv_set) * KCAL_PER_HARTREE < etol or int(b) == int(e)
                 orbconv = abs(energy - eprv_orb) * KCAL_PER_HARTREE < etol or nset == 1
+        e_w = ctx.energy(c)
         return OptResult(energy=energy, coeff=c, coeff_sc=np.array(s.coeff_sc, dtype=np.float64), iterations=calls,
-                         gradient_norm=float("nan"), history=hist)
+                         gradient_norm=float("nan"), history=hist, energy_at_weights=e_w)

@@ -342,11 +342,13 @@ def read_input(path: str) -> VsvbSystem:
         return parse_input(fh.read())
 
 
-def write_input(s: VsvbSystem) -> str:
+def write_input(s: VsvbSystem, orbitals: bool = True) -> str:
     """Write ``s`` in the reference's input format (inverse of parse_input).
 
     Coordinates go out in Angstrom with 17 significant digits so a
-    parse/write/parse round trip is exact.
+    parse/write/parse round trip is exact.  orbitals=False stops before the
+    orbital records (and the title), for a deck completed by an 'orbitals'
+    file (format_orbitals), as the reference's `cat input orbitals` does.
     """
     out: List[str] = []
     nxorb = len(s.xorb)
@@ -380,6 +382,8 @@ def write_input(s: VsvbSystem) -> str:
                 out.append(f"{float(s.coeff_sc[j])!r} " + " ".join(f"{a} {b}" for a, b in s.pair_sc[j]))
     if nxorb:
         out.append(" ".join(f"{a} {b}" for a, b in zip(s.xorb, s.xroot)))
+    if not orbitals:
+        return "\n".join(out) + "\n"
     for p in range(s.norbs):
         na = s.orb_natom[p]
         o0, o1 = s.orb_start[p], s.orb_start[p + 1]
@@ -389,6 +393,132 @@ def write_input(s: VsvbSystem) -> str:
     if s.title:
         out.append(s.title)
     return "\n".join(out) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# The 'orbitals' and 'nelecwfn' files (xm_output, src/xm_module.F90:464-577).
+# The reference writes them with Fortran formats; their text is restated here
+# field for field so that a file from either side reads the same:
+#   1  (5x,1i2,5x,10i4)          orbital header: natoms, atoms, nterm
+#   2  (4(1i4,1x,1f13.8))        orbital terms: AO index, weight (4 a record)
+#   3  (1x,7(2i3,2x))            orbital pairs (one coupling) / excited roots
+#   4  (1x,1f13.8,7(2i3,2x))     coupling weight and orbital pairs
+#   5  (/,1x,'total energy in atomic units ',1f32.16)
+#   7  (1x,'converged to ',1e10.2,' kCal/mol')
+# Format reversion (more items than one pass of a format takes) starts a new
+# record at the last top-level group, or at the beginning when there is none;
+# position edits after the last item write nothing (gfortran).
+# ---------------------------------------------------------------------------
+
+def _fi(v: int, w: int) -> str:
+    t = str(int(v))
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def _ff(v: float, w: int, d: int) -> str:
+    t = f"{float(v):{w}.{d}f}"
+    return t if len(t) <= w else "*" * w
+
+
+def _fe(v: float, w: int, d: int) -> str:
+    """Fortran Ew.d: sign, 0.ddd, E, signed two-digit exponent, right-justified."""
+    x = float(v)
+    if x == 0.0:
+        mant, ex = 0, 0
+    else:
+        ex = int(np.floor(np.log10(abs(x)))) + 1
+        mant = int(round(abs(x) / 10.0 ** ex * 10 ** d))
+        if mant >= 10 ** d:  # rounding carried into a new digit
+            mant //= 10
+            ex += 1
+    t = ("-" if x < 0 else "") + "0." + str(mant).rjust(d, "0") + "E" + ("-" if ex < 0 else "+") + f"{abs(ex):02d}"
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def _fmt_header(items: Sequence[int]) -> List[str]:
+    """Format 1: each record takes one item as i2 and then up to ten as i4."""
+    lines, k = [], 0
+    while k < len(items):
+        line = "     " + _fi(items[k], 2)
+        k += 1
+        if k < len(items):
+            line += "     " + "".join(_fi(x, 4) for x in items[k:k + 10])
+            k += 10
+        lines.append(line)
+    return lines
+
+
+def _fmt_terms(idx: Sequence[int], w: Sequence[float]) -> List[str]:
+    """Format 2: (i4, 1x, f13.8) pairs, four to a record."""
+    pairs = [_fi(a, 4) + " " + _ff(b, 13, 8) for a, b in zip(idx, w)]
+    return ["".join(pairs[k:k + 4]) for k in range(0, len(pairs), 4)]
+
+
+def _fmt_pairs(lead: str, pairs: Sequence[Sequence[int]]) -> List[str]:
+    """Formats 3 and 4: ``lead`` (1x, or 1x and a weight), then (2i3, 2x)
+    groups, seven to a record; continuation records repeat only the group."""
+    groups = [_fi(a, 3) + _fi(b, 3) for a, b in pairs]
+    if not groups:
+        return [lead.rstrip()]
+    lines = []
+    for k in range(0, len(groups), 7):
+        lines.append((lead if k == 0 else "") + "  ".join(groups[k:k + 7]))
+    return lines
+
+
+def format_nelecwfn(s: VsvbSystem) -> str:
+    """The 'nelecwfn' file the reference writes when nspinc > 1 (xm_module.F90:480-503):
+    every coupling's weight and orbital pairs, then the excited-orbital roots,
+    then a blank record.  Empty for nspinc <= 1 (the reference writes none)."""
+    if s.nspinc <= 1:
+        return ""
+    out: List[str] = []
+    for j in range(s.nspinc):
+        out += _fmt_pairs(" " + _ff(s.coeff_sc[j], 13, 8), [tuple(p) for p in s.pair_sc[j]])
+    out += _fmt_pairs(" ", list(zip(s.xorb, s.xroot)))
+    out.append("")
+    return "\n".join(out) + "\n"
+
+
+def format_orbitals(s: VsvbSystem, energy: float, mode: str = "save", etol: float = 0.0) -> str:
+    """The 'orbitals' file (xm_module.F90:506-570): every orbital (spin-coupled,
+    unpaired, doubly occupied), a blank record, the NDFs, then the total energy,
+    with mode 'done' the convergence tolerance etol (kcal/mol), and a blank
+    record.  The orbital records are the orbital part of an input deck, which
+    is how the reference continues a run from them (valence_api.F90:70-80)."""
+    out: List[str] = []
+    nreg = 2 * s.npair + s.nunpd + s.ndocc
+
+    def orbital(p: int):
+        na = int(s.orb_natom[p])
+        o0, o1 = int(s.orb_start[p]), int(s.orb_start[p + 1])
+        out.extend(_fmt_header([na] + [int(a) + 1 for a in s.orb_atoms[p, :na]] + [o1 - o0]))
+        out.extend(_fmt_terms(s.xpset[o0:o1], s.coeff[o0:o1]))
+
+    for p in range(nreg):
+        orbital(p)
+    out.append("")
+    for p in range(nreg, nreg + s.ndf):
+        orbital(p)
+    out.append("")
+    out.append(" total energy in atomic units " + _ff(energy, 32, 16))
+    if mode == "done":
+        out.append(" converged to " + _fe(etol, 10, 2) + " kCal/mol")
+    out.append("")
+    return "\n".join(out) + "\n"
+
+
+def write_orbitals(s: VsvbSystem, energy: float, directory: str = ".", mode: str = "save", etol: float = 0.0,
+                   append: bool = False) -> None:
+    """xm_output: 'orbitals' (and 'nelecwfn' when nspinc > 1) in ``directory``,
+    replaced or (append) extended."""
+    import os
+    how = "a" if append else "w"
+    if s.nspinc > 1:
+        with open(os.path.join(directory, "nelecwfn"), how) as fh:
+            fh.write(format_nelecwfn(s))
+    with open(os.path.join(directory, "orbitals"), how) as fh:
+        fh.write(format_orbitals(s, energy, mode, etol))
 
 
 def _xpmax(s: VsvbSystem) -> int:
