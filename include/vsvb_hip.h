@@ -71,6 +71,12 @@ typedef struct vsvb_options {
                                * grid-wide elimination (one pair of launches per pivot), smaller ones by one
                                * workgroup: 0 = default (129), else >= 1 (1 = always grid-wide).  Results
                                * do not depend on it beyond rounding. */
+    int32_t beta_only_last;   /* 1: the last doubly occupied orbital holds only a beta electron, so that the
+                               * alpha and beta orbital sets differ in one orbital -- the determinant of the
+                               * first-order orbital step, whose alpha copy of a doubly occupied orbital is
+                               * substituted (src/valence.F90:566-590): the substitute is the last unpaired
+                               * orbital, the original the last doubly occupied one.  Needs npair = 0,
+                               * nunpd >= 1, ndocc >= 1.  0 (default): the reference's wave function. */
 } vsvb_options;
 
 typedef struct vsvb_context vsvb_context;

@@ -32,6 +32,8 @@ program vsvb_drive
      print '(a,i0)', 'sizeof vsvb_options ', c_sizeof(opt)
      print '(a,i0)', 'offset grid_inverse_min ', &
           transfer(c_loc(opt%grid_inverse_min), 0_c_intptr_t) - transfer(c_loc(opt), 0_c_intptr_t)
+     print '(a,i0)', 'offset beta_only_last ', &
+          transfer(c_loc(opt%beta_only_last), 0_c_intptr_t) - transfer(c_loc(opt), 0_c_intptr_t)
      stop
   end if
 

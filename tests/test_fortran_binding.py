@@ -23,6 +23,7 @@ int main(void) {
   printf("offset coeff_sc %zu\n", offsetof(vsvb_system, coeff_sc));
   printf("sizeof vsvb_options %zu\n", sizeof(vsvb_options));
   printf("offset grid_inverse_min %zu\n", offsetof(vsvb_options, grid_inverse_min));
+  printf("offset beta_only_last %zu\n", offsetof(vsvb_options, beta_only_last));
   return 0;
 }
 """
@@ -53,4 +54,4 @@ def test_fortran_binding_builds_and_matches_c_layout(tmp_path):
     lf, lc = _layout(f.stdout), _layout(c.stdout)
     print("Fortran:", lf)
     print("C:      ", lc)
-    assert len(lc) == 6 and lf == lc
+    assert len(lc) == 7 and lf == lc

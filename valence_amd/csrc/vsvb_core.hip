@@ -1119,6 +1119,13 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->norbs = s->norbs;
   c->totlen = s->totlen;
   c->nsp = 2 * s->npair + s->nunpd + s->ndocc;
+  c->nalpha_rows = c->nsp;
+  if (opt && opt->beta_only_last) {
+    if (opt->beta_only_last != 1) return fail(VSVB_ERR_ARG, "beta_only_last must be 0 or 1");
+    if (s->npair != 0 || s->nunpd < 1 || s->ndocc < 1)
+      return fail(VSVB_ERR_UNSUPPORTED, "beta_only_last needs npair = 0, nunpd >= 1 and ndocc >= 1");
+    c->nalpha_rows = c->nsp - 1;
+  }
   c->first_ndf = s->norbs - s->ndf;
   c->dtol = std::pow(10.0, -(double)s->ntol_d);
   if (c->nsp < 1) return fail(VSVB_ERR_ARG, "no electrons");
@@ -1459,7 +1466,7 @@ static int prepare_orbitals(vsvb_context* c, const double* coeff_host, const dou
   const int nspin = c->nunpd > 0 ? 2 : 1;
   for (int s = 0; s < nspin; ++s) {
     const int off = s == 0 ? 0 : c->nunpd;
-    const int m = s == 0 ? ns : c->ndocc;
+    const int m = s == 0 ? c->nalpha_rows : c->ndocc;
     double* P = c->d_P.p + (size_t)s * ns * ns;
     if (m == 0) continue;
     hipLaunchKernelGGL(k_copy_block, dim3((m * m + 255) / 256), dim3(256), 0, st, c->d_Sorb.p, ns, off, m, P);
@@ -1723,7 +1730,7 @@ static int gradient_impl(vsvb_context* c, double* grad_dev) {
   hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->d_gradC.p, (size_t)ns * n, 0.0);
   for (int s = 0; s < nspin; ++s) {
     const int off = s == 0 ? 0 : c->nunpd;
-    const int m = s == 0 ? ns : c->ndocc;
+    const int m = s == 0 ? c->nalpha_rows : c->ndocc;
     if (m == 0) continue;
     double* F = c->d_F.p + (size_t)s * nn;
     hipLaunchKernelGGL(k_sym_combine, dim3(gb), dim3(T), 0, st, c->d_K.p + (size_t)s * nn, -1.0, c->d_Jsym.p, cfac,

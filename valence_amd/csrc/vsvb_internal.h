@@ -154,6 +154,8 @@ struct vsvb_context {
   // sizes
   int natom = 0, nsh = 0, nbf = 0, npairs = 0, npp = 0, nsp = 0, norbs = 0, totlen = 0;
   int npair = 0, nunpd = 0, ndocc = 0, ndf = 0, first_ndf = 0;
+  int nalpha_rows = 0;  // spatial orbitals with an alpha electron: rows [0, nalpha_rows) of C (nsp, or
+                        // nsp - 1 when the last doubly occupied orbital holds only a beta electron)
   double dtol = 0.0;
   int pair_off[vsvb::kNPairClass + 1] = {0};
   // host copies

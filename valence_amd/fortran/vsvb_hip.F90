@@ -25,6 +25,7 @@ module vsvb_hip
      integer(c_int32_t) :: device = 0
      integer(c_int32_t) :: pair_chunk = 0             ! 0: claim width from the system size
      integer(c_int32_t) :: grid_inverse_min = 0       ! 0: overlap blocks from 129 rows up inverted grid-wide
+     integer(c_int32_t) :: beta_only_last = 0         ! 1: last docc orbital beta only (first-order step)
   end type vsvb_options_t
 
   interface

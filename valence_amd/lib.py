@@ -41,7 +41,7 @@ class CSystem(C.Structure):
 
 class COptions(C.Structure):
     _fields_ = [("quartet_tau", C.c_double), ("device", C.c_int32), ("pair_chunk", C.c_int32),
-                ("grid_inverse_min", C.c_int32)]
+                ("grid_inverse_min", C.c_int32), ("beta_only_last", C.c_int32)]
 
 
 EXPORTS = ["vsvb_context_create", "vsvb_context_destroy", "vsvb_energy", "vsvb_energy_parts", "vsvb_energy_grad",
@@ -167,12 +167,12 @@ class Context:
     """One molecule/basis/orbital layout resident on the GPU (vsvb_context)."""
 
     def __init__(self, s: VsvbSystem, quartet_tau: float = 0.0, device: int = 0, pair_chunk: int = 0,
-                 grid_inverse_min: int = 0):
+                 grid_inverse_min: int = 0, beta_only_last: bool = False):
         L = load()
         self.sys = s
         self._packed = _Packed(s)
         opt = COptions(quartet_tau=quartet_tau, device=device, pair_chunk=pair_chunk,
-                       grid_inverse_min=grid_inverse_min)
+                       grid_inverse_min=grid_inverse_min, beta_only_last=int(beta_only_last))
         h = C.c_void_p()
         _check(L.vsvb_context_create(C.byref(self._packed.cs), C.byref(opt), C.byref(h)), "vsvb_context_create")
         self._h = h

@@ -36,7 +36,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import lib
-from .system import VsvbSystem, write_orbitals
+from .system import VsvbSystem, nuclear_repulsion, write_orbitals
 
 KCAL_PER_HARTREE = 627.509469  # the reference's tokcal (src/valence.F90:2759)
 
@@ -52,6 +52,248 @@ class OptResult:
     # the energy of the returned weights; differs from ``energy`` after a DEM
     # whose last trials failed (see dem_minimize)
     energy_at_weights: float = float("nan")
+
+
+# ---------------------------------------------------------------------------
+# First-order orbital step (first_order_opt, src/valence.F90:527-844)
+# ---------------------------------------------------------------------------
+# The reference replaces orbital iorb, in the bra and in the ket, by each
+# function of its optimization space in turn (an AO, or an NDF where its
+# weight refers to one), evaluates <Psi_i|H|Psi_j> and <Psi_i|Psi_j> for every
+# pair (ham, ovl; :674-704), solves ham w = E ovl w (rsg) and takes the lowest
+# root, or root(i) + 1 for an excited orbital xorb(i) (:790-805), as the new
+# weights; the eigenvalue plus E_nuc is the new energy.  A doubly occupied
+# orbital is made singly occupied for this: only its alpha copy is replaced
+# (:566-590), and with unpaired electrons the beta-replaced matrices are added
+# (spin averaging, :706-748).
+#
+# The wave function is linear in the replaced orbital's weights x, so its
+# energy is the Rayleigh quotient E(x) = E_nuc + x^T H x / x^T S x of the
+# pencil (H, S) = (ham, ovl) up to a common factor, and the library's exact
+# gradient satisfies, at any x,
+#     H x - E_el(x) S x - (x^T S x) grad_x E / 2 = 0,
+# n equations that are linear in the unknown (H, S).  Evaluated at n + 4
+# points they fix (H, S) up to that common factor (the null vector of the
+# stacked equations), which is all the eigenproblem needs.  A doubly occupied
+# orbital is evaluated on a copy of the system in which the replacement is a
+# separate orbital holding the alpha electron and the original keeps only the
+# beta one (vsvb_options.beta_only_last).  Spin-averaged pencils are each
+# scaled to x^T S x = 1 at the current weights before they are added: both
+# determinants then have the same norm there, so the sum matches the
+# reference's up to one common factor.
+
+
+def _pencil_from_gradients(eval_fn, c: np.ndarray, seed: int = 0):
+    """(H, S), symmetric, with c^T S c = 1, of E_el(x) = x^T H x / x^T S x from
+    eval_fn(x) -> (E_el, dE/dx) at n + 4 seeded points around c (not at c
+    itself, where a spin-coupled expansion may have a singular transition
+    overlap).  The solution is the null vector of the stacked equations; it
+    must stand out from the next singular value by a factor of 1e5 at least
+    (weights nearly spanned by the other orbitals make the equations
+    ill-conditioned along those directions, not ambiguous)."""
+    n = c.size
+    iu, ju = np.triu_indices(n)
+    nv = iu.size
+    rng = np.random.default_rng(seed)
+    scale = np.linalg.norm(c) / np.sqrt(n)
+    pts = [c + 0.5 * scale * rng.standard_normal(n) for _ in range(n + 4)]
+    rows = []
+    for x in pts:
+        e, g = eval_fn(x)
+        # coefficient of H_ij / S_ij (i <= j) in (H x)_r and in x^T S x
+        A_h = np.zeros((n, nv))
+        A_h[iu, np.arange(nv)] += x[ju]
+        off = iu != ju
+        A_h[ju[off], np.arange(nv)[off]] += x[iu[off]]
+        quad = np.where(off, 2.0, 1.0) * x[iu] * x[ju]
+        A_s = -e * A_h - 0.5 * np.outer(g, quad)
+        blk = np.hstack([A_h, A_s])
+        blk /= np.linalg.norm(blk, axis=1, keepdims=True) + 1e-300
+        rows.append(blk)
+    A = np.vstack(rows)
+    _, sv, vt = np.linalg.svd(A, full_matrices=False)
+    if not sv[-1] <= 1e-5 * sv[-2]:
+        raise RuntimeError(f"orbital pencil not determined by the energies and gradients "
+                           f"(singular values {sv[-2]:.3e}, {sv[-1]:.3e})")
+    z = vt[-1]
+    H = np.zeros((n, n)); S = np.zeros((n, n))
+    H[iu, ju] = z[:nv]; H[ju, iu] = z[:nv]
+    S[iu, ju] = z[nv:]; S[ju, iu] = z[nv:]
+    nrm = float(c @ S @ c)
+    return H / nrm, S / nrm
+
+
+def _substituted(s: VsvbSystem, p: int, beta: bool):
+    """Copy of closed- or high-spin ``s`` in which doubly occupied orbital p is
+    split: a replacement x (same optimization space) holds the alpha electron
+    and p the beta one (beta: the other way round).  Orbital order: unpaired,
+    then the alpha holder (as the last unpaired orbital), the other doubly
+    occupied ones, the beta holder (as the last doubly occupied orbital), the
+    NDFs -- the layout vsvb_options.beta_only_last describes.  Returns the
+    system and assemble(c, x): its weight vector from ``s``'s weights c and x."""
+    nu, nd = int(s.nunpd), int(s.ndocc)
+    dl = list(range(nu, nu + nd))
+    dl.remove(p)
+    X = -1  # the replacement
+    order = list(range(nu)) + [p if beta else X] + dl + [X if beta else p] + list(range(nu + nd, s.norbs))
+    t = s.copy()
+    natom, atoms, start, xp, src = [], [], [0], [], []
+    for q in order:
+        o = p if q == X else q
+        sl = slice(int(s.orb_start[o]), int(s.orb_start[o + 1]))
+        natom.append(int(s.orb_natom[o]))
+        atoms.append(s.orb_atoms[o])
+        xp.extend(int(v) for v in s.xpset[sl])
+        start.append(start[-1] + sl.stop - sl.start)
+        src.append((q, sl))
+    t.nunpd = nu + 1
+    t.orb_natom = np.array(natom, np.int32)
+    t.orb_atoms = np.array(atoms, np.int32)
+    t.orb_start = np.array(start, np.int32)
+    t.xpset = np.array(xp, np.int32)
+    t.coeff = np.zeros(len(xp))
+    t.xorb = np.zeros(0, np.int32)
+    t.xroot = np.zeros(0, np.int32)
+    t.orbset = np.zeros((0, 2), np.int32)
+
+    def assemble(c, x):
+        return np.concatenate([x if q == X else c[sl] for q, sl in src])
+
+    xs = next(slice(int(t.orb_start[k]), int(t.orb_start[k + 1])) for k, (q, _) in enumerate(src) if q == X)
+    return t, assemble, xs
+
+
+def first_order_matrices(s: VsvbSystem, c: np.ndarray, iorb: int, device: int = 0, contexts: Optional[dict] = None):
+    """(ham, ovl) of first_order_opt for orbital iorb (1-based) at weights c, up
+    to a common factor (scaled to c_iorb^T ovl c_iorb = 1 per spin-averaged
+    part), and E_nuc.  ``contexts`` caches the library contexts between calls."""
+    p = iorb - 1
+    nnd = 2 * int(s.npair) + int(s.nunpd)
+    nsp = nnd + int(s.ndocc)
+    if not 0 <= p < nsp:
+        raise ValueError(f"orbital {iorb} is not an occupied orbital")
+    sl = slice(int(s.orb_start[p]), int(s.orb_start[p + 1]))
+    cache = {} if contexts is None else contexts
+    parts = []
+    if p < nnd:  # singly occupied (spin-coupled or unpaired): linear in its weights as it is
+        key = ("own", p)
+        if key not in cache:
+            cache[key] = (lib.Context(s, device=device), None, sl)
+        variants = [key]
+    else:
+        if s.npair > 0:
+            raise NotImplementedError("first-order step of a doubly occupied orbital in a spin-coupled wave "
+                                      "function")
+        variants = [("sub", p, False)] + ([("sub", p, True)] if s.nunpd > 0 else [])
+        for key in variants:
+            if key not in cache:
+                t, assemble, xs = _substituted(s, p, key[2])
+                t.coeff = assemble(c, np.array(c[sl]))
+                cache[key] = (lib.Context(t, device=device, beta_only_last=True), assemble, xs)
+    H = S = None
+    enuc = nuclear_repulsion(s)
+    for key in variants:
+        ctx, assemble, xs = cache[key]
+
+        def eval_fn(x, ctx=ctx, assemble=assemble, xs=xs):
+            if assemble is None:
+                full = c.copy()
+                full[xs] = x
+            else:
+                full = assemble(c, x)
+            e, g = ctx.energy_grad(full)
+            return e - enuc, g[xs]
+
+        Hk, Sk = _pencil_from_gradients(eval_fn, np.array(c[sl], dtype=np.float64))
+        H = Hk if H is None else H + Hk
+        S = Sk if S is None else S + Sk
+    return H, S, enuc
+
+
+def first_order_opt(s: VsvbSystem, c: np.ndarray, iorb: int, device: int = 0,
+                    contexts: Optional[dict] = None, norm_ctx=None):
+    """One first-order step of orbital iorb (1-based): new weights (normalized,
+    normal) in a copy of c and the step's energy (eigenvalue + E_nuc)."""
+    from scipy.linalg import eigh
+    H, S, enuc = first_order_matrices(s, c, iorb, device, contexts)
+    w, V = eigh(H, S)
+    root = 0
+    for xo, xr in zip(s.xorb, s.xroot):
+        if int(xo) == iorb:
+            root = int(xr)
+    p = iorb - 1
+    sl = slice(int(s.orb_start[p]), int(s.orb_start[p + 1]))
+    out = np.array(c, dtype=np.float64)
+    out[sl] = V[:, root]
+    if norm_ctx is not None:  # normal(iorb, iorb): unit self-overlap
+        norm_ctx.energy(out)
+        Sd = norm_ctx.buffer("S_orb", s.nspatial * s.nspatial).reshape(s.nspatial, s.nspatial)
+        out[sl] /= np.sqrt(Sd[p, p])
+    return out, float(w[root]) + enuc
+
+
+def minimize_energy_first_order(s: VsvbSystem, device: int = 0) -> OptResult:
+    """The reference's minimize_energy (src/valence.F90:2744-2885) with its
+    first-order method (ptbnmax = 0): tolerance sweeps ntol_e_min..ntol_e_max,
+    the orbital sets (orbset) optimized orbital by orbital until the energy
+    changes by less than 10^-ntol kcal/mol, spin_opt after each sweep when
+    nspinc > 1, at most max_iter iterations.  ``energy`` is the energy the
+    reference tracks (the last step's eigenvalue + E_nuc); ``energy_at_weights``
+    that of the returned weights."""
+    c = np.array(s.coeff, dtype=np.float64)
+    csc = np.array(s.coeff_sc, dtype=np.float64)
+    hist: List[float] = []
+    contexts: dict = {}
+    ctx = lib.Context(s, device=device)
+    try:
+        # calculate_vsvb_energy normalizes every orbital before the guess energy (:144-145)
+        energy = ctx.energy(c)
+        n = s.nspatial
+        Sd = ctx.buffer("S_orb", n * n).reshape(n, n)
+        for p in range(s.norbs - s.ndf):
+            c[int(s.orb_start[p]):int(s.orb_start[p + 1])] /= np.sqrt(Sd[p, p])
+        hist.append(energy)
+        num_iter = 0
+        nset = len(s.orbset)
+        for ntol in range(int(s.ntol_e_min), int(s.ntol_e_max) + 1):
+            etol = 10.0 ** -ntol
+            finished = False
+            while not finished:
+                eprv_sc = energy
+                orbconv = False
+                while not orbconv and num_iter < s.max_iter:
+                    eprv_orb = energy
+                    for b, e in s.orbset:
+                        setcnv = False
+                        while not setcnv and num_iter < s.max_iter:
+                            eprv_set = energy
+                            num_iter += 1
+                            for iorb in range(int(b), int(e) + 1):
+                                c, energy = first_order_opt(s, c, iorb, device, contexts, norm_ctx=ctx)
+                            hist.append(energy)
+                            setcnv = abs(energy - eprv_set) * KCAL_PER_HARTREE < etol
+                    orbconv = abs(energy - eprv_orb) * KCAL_PER_HARTREE < etol
+                    if nset == 1:
+                        orbconv = True
+                if s.nspinc > 1:
+                    num_iter += 1
+                    energy, csc = ctx.spin_opt(c)
+                    ctx.set_spin_coupling(csc)
+                    for key in list(contexts):
+                        contexts[key][0].set_spin_coupling(csc)
+                    hist.append(energy)
+                    finished = abs(energy - eprv_sc) * KCAL_PER_HARTREE < etol
+                else:
+                    finished = True
+                if num_iter >= s.max_iter:
+                    finished = True
+        e_w = ctx.energy(c)
+        return OptResult(energy=energy, coeff=c, coeff_sc=csc, iterations=num_iter, gradient_norm=float("nan"),
+                         history=hist, energy_at_weights=e_w)
+    finally:
+        ctx.close()
+        for v in contexts.values():
+            v[0].close()
 
 
 def save_orbitals(s: VsvbSystem, r: "OptResult", directory: str, mode: str = "done") -> None:
