@@ -75,8 +75,8 @@ typedef struct vsvb_options {
                                * alpha and beta orbital sets differ in one orbital -- the determinant of the
                                * first-order orbital step, whose alpha copy of a doubly occupied orbital is
                                * substituted (src/valence.F90:566-590): the substitute is the last unpaired
-                               * orbital, the original the last doubly occupied one.  Needs npair = 0,
-                               * nunpd >= 1, ndocc >= 1.  0 (default): the reference's wave function. */
+                               * orbital, the original the last doubly occupied one.  Needs nunpd >= 1
+                               * and ndocc >= 1.  0 (default): the reference's wave function. */
 } vsvb_options;
 
 typedef struct vsvb_context vsvb_context;

@@ -64,17 +64,23 @@ def test_pencil_rejects_inconsistent_gradients():
 
 
 @pytest.mark.parametrize("beta", [False, True])
-def test_split_system_layout(beta):
-    s = read_input(os.path.join(GOLDEN, "inputs", "testing__c2-pt-vshf-p2"))  # unpaired, docc and NDFs
-    assert s.nunpd == 2 and s.ndocc == 5 and s.ndf == 2
-    p = s.nunpd + 2                                         # the third doubly occupied orbital
+@pytest.mark.parametrize("deck,k", [("testing__c2-pt-vshf-p2", 2), ("testing__lih-sv", 0)])
+def test_split_system_layout(beta, deck, k):
+    # c2-pt-vshf-p2: unpaired, doubly occupied and NDF orbitals; lih-sv: a spin-coupled pair first
+    s = read_input(os.path.join(GOLDEN, "inputs", deck))
+    nnd = 2 * s.npair + s.nunpd
+    p = nnd + k                                             # doubly occupied orbital k
     t, assemble, xs = _substituted(s, p, beta)
-    assert t.nunpd == s.nunpd + 1 and t.ndocc == s.ndocc and t.norbs == s.norbs + 1
+    assert t.nunpd == s.nunpd + 1 and t.ndocc == s.ndocc and t.norbs == s.norbs + 1 and t.npair == s.npair
     t.coeff = assemble(s.coeff, np.arange(xs.stop - xs.start, dtype=float) + 100.0)
     t.validate()
+    for q in range(2 * s.npair):                            # spin-coupled orbitals keep their place
+        a = slice(int(s.orb_start[q]), int(s.orb_start[q + 1]))
+        b = slice(int(t.orb_start[q]), int(t.orb_start[q + 1]))
+        assert np.array_equal(s.coeff[a], t.coeff[b])
     sl = slice(int(s.orb_start[p]), int(s.orb_start[p + 1]))
-    x_pos = s.nunpd + s.ndocc if beta else s.nunpd         # the replacement's orbital index in t
-    p_pos = s.nunpd if beta else s.nunpd + s.ndocc         # the original's
+    x_pos = nnd + s.ndocc if beta else nnd                  # the replacement's orbital index in t
+    p_pos = nnd if beta else nnd + s.ndocc                  # the original's
     for q, w in ((x_pos, 100.0 + np.arange(sl.stop - sl.start)), (p_pos, s.coeff[sl])):
         tq = slice(int(t.orb_start[q]), int(t.orb_start[q + 1]))
         assert np.array_equal(t.coeff[tq], w)

@@ -6,9 +6,8 @@ the converged totals the reference records for its first-order inputs
 (unpaired orbitals; doubly occupied ones with spin averaging), NDF-expanded
 orbitals, excited orbitals (xorb roots) and spin-coupled wave functions with
 their spin-coupling optimization.  testing/c2-pt-vshf-p2 stops at max_iter =
-10, so it also pins the iteration itself; testing/lih-sv would need the step
-of a doubly occupied orbital inside a spin-coupled wave function, which is
-not restated (NotImplementedError).
+10, so it also pins the iteration itself; testing/lih-sv optimizes a doubly
+occupied orbital inside a spin-coupled wave function.
 """
 import json
 import os
@@ -25,7 +24,7 @@ KNOWN = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
 CASES = ["testing/be", "testing/he", "testing/h", "testing/h+ndf", "testing/be+ndf", "testing/h2o-vdz",
          "testing/be3s2", "testing/he3s-1s3s", "examples/li_opt", "testing/c2-pt-vshf-p2", "testing/cu+",
          "testing/h2-sz", "testing/h2-dz", "testing/be-sc", "testing/he1s2s", "testing/h2o-vdz-sc1",
-         "testing/be-scv3s+2sc"]
+         "testing/be-scv3s+2sc", "testing/lih-sv"]
 
 
 def _case(name):
@@ -47,10 +46,3 @@ def test_first_order_matches_reference(name):
           f"E(weights)={r.energy_at_weights:.12f}")
     assert abs(r.history[0] - rec["guess_energy"]) < 1e-8
     assert abs(d) < 1e-6
-
-
-def test_first_order_docc_in_spin_coupled_is_refused():
-    from valence_amd.optimize import minimize_energy_first_order
-    s, _ = _case("testing/lih-sv")
-    with pytest.raises(NotImplementedError):
-        minimize_energy_first_order(s)

@@ -1122,8 +1122,8 @@ static int build_context(const vsvb_system* s, const vsvb_options* opt, vsvb_con
   c->nalpha_rows = c->nsp;
   if (opt && opt->beta_only_last) {
     if (opt->beta_only_last != 1) return fail(VSVB_ERR_ARG, "beta_only_last must be 0 or 1");
-    if (s->npair != 0 || s->nunpd < 1 || s->ndocc < 1)
-      return fail(VSVB_ERR_UNSUPPORTED, "beta_only_last needs npair = 0, nunpd >= 1 and ndocc >= 1");
+    if (s->nunpd < 1 || s->ndocc < 1)
+      return fail(VSVB_ERR_UNSUPPORTED, "beta_only_last needs nunpd >= 1 and ndocc >= 1");
     c->nalpha_rows = c->nsp - 1;
   }
   c->first_ndf = s->norbs - s->ndf;

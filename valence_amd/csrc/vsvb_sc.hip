@@ -567,7 +567,9 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
   if (s->nspinc < 1 || !s->pair_sc || !s->coeff_sc) return fail(VSVB_ERR_ARG, "spin-coupled pairs without spin couplings");
   if (s->npair > 16) return fail(VSVB_ERR_UNSUPPORTED, "more than 16 spin-coupled pairs");
   const int np = s->npair, nnd = 2 * s->npair + s->nunpd;
-  sc.na = s->npair + s->nunpd + s->ndocc;
+  // doubly occupied orbitals with an alpha electron (all, or all but the last: beta_only_last)
+  const int nda = s->ndocc - (c->nalpha_rows < c->nsp ? 1 : 0);
+  sc.na = s->npair + s->nunpd + nda;
   sc.nb = s->npair + s->ndocc;
   if (sc.na > kMaxSpinDim) return fail(VSVB_ERR_UNSUPPORTED, "alpha block larger than 48 orbitals");
   sc.ndet = s->nspinc << np;
@@ -598,7 +600,7 @@ int vsvb::sc_setup(vsvb_context* c, const vsvb_system* s) {
       }
       for (int u = 0; u < s->nunpd; ++u) A[np + u] = 2 * np + u;
       for (int o = 0; o < s->ndocc; ++o) {
-        A[np + s->nunpd + o] = nnd + o;
+        if (o < nda) A[np + s->nunpd + o] = nnd + o;
         B[np + o] = nnd + o;
       }
       cd[d] = s->coeff_sc[isc];
@@ -780,7 +782,8 @@ int vsvb::sc_matrices(vsvb_context* c, double* ham, double* ovl) {
   // determinant pair twice (bra and ket), a doubly occupied one four times.
   const int nnd = 2 * sc.npair + c->nunpd;
   double logf = 0.0;
-  for (int p = 0; p < ns; ++p) logf -= (p < nnd ? 1.0 : 2.0) * std::log(sorb[(size_t)p * ns + p]);
+  for (int p = 0; p < ns; ++p)
+    logf -= (p < nnd || p >= c->nalpha_rows ? 1.0 : 2.0) * std::log(sorb[(size_t)p * ns + p]);
   const double f = std::exp(logf);
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < m; ++j) {

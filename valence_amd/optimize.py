@@ -124,18 +124,19 @@ def _pencil_from_gradients(eval_fn, c: np.ndarray, seed: int = 0):
 
 
 def _substituted(s: VsvbSystem, p: int, beta: bool):
-    """Copy of closed- or high-spin ``s`` in which doubly occupied orbital p is
-    split: a replacement x (same optimization space) holds the alpha electron
-    and p the beta one (beta: the other way round).  Orbital order: unpaired,
-    then the alpha holder (as the last unpaired orbital), the other doubly
-    occupied ones, the beta holder (as the last doubly occupied orbital), the
-    NDFs -- the layout vsvb_options.beta_only_last describes.  Returns the
-    system and assemble(c, x): its weight vector from ``s``'s weights c and x."""
-    nu, nd = int(s.nunpd), int(s.ndocc)
-    dl = list(range(nu, nu + nd))
+    """Copy of ``s`` in which doubly occupied orbital p is split: a replacement
+    x (same optimization space) holds the alpha electron and p the beta one
+    (beta: the other way round).  Orbital order: spin-coupled and unpaired
+    orbitals, then the alpha holder (as the last unpaired orbital), the other
+    doubly occupied ones, the beta holder (as the last doubly occupied
+    orbital), the NDFs -- the layout vsvb_options.beta_only_last describes.
+    Returns the system, assemble(c, x) (its weight vector from ``s``'s weights
+    c and x) and the slice of x in it."""
+    nnd, nd = 2 * int(s.npair) + int(s.nunpd), int(s.ndocc)
+    dl = list(range(nnd, nnd + nd))
     dl.remove(p)
     X = -1  # the replacement
-    order = list(range(nu)) + [p if beta else X] + dl + [X if beta else p] + list(range(nu + nd, s.norbs))
+    order = list(range(nnd)) + [p if beta else X] + dl + [X if beta else p] + list(range(nnd + nd, s.norbs))
     t = s.copy()
     natom, atoms, start, xp, src = [], [], [0], [], []
     for q in order:
@@ -146,7 +147,7 @@ def _substituted(s: VsvbSystem, p: int, beta: bool):
         xp.extend(int(v) for v in s.xpset[sl])
         start.append(start[-1] + sl.stop - sl.start)
         src.append((q, sl))
-    t.nunpd = nu + 1
+    t.nunpd = int(s.nunpd) + 1
     t.orb_natom = np.array(natom, np.int32)
     t.orb_atoms = np.array(atoms, np.int32)
     t.orb_start = np.array(start, np.int32)
@@ -181,9 +182,6 @@ def first_order_matrices(s: VsvbSystem, c: np.ndarray, iorb: int, device: int = 
             cache[key] = (lib.Context(s, device=device), None, sl)
         variants = [key]
     else:
-        if s.npair > 0:
-            raise NotImplementedError("first-order step of a doubly occupied orbital in a spin-coupled wave "
-                                      "function")
         variants = [("sub", p, False)] + ([("sub", p, True)] if s.nunpd > 0 else [])
         for key in variants:
             if key not in cache:
