@@ -5,7 +5,7 @@ energy (1e-9 Eh) and its gradient with central differences of the oracle
 energy (h = 1e-5, 1e-7 relative).  The 128-atom 4x4x8 slab (1536 AOs, 384 docc) is beyond the oracle's
 reach, so it is checked through properties that hold at any size: invariance
 to scaling any orbital, to mixing orbitals that share an atom, and to a rigid
-translation (all 1e-9 relative), and its gradient through the scaling identity
+translation (1e-9 Eh absolute), and its gradient through the scaling identity
 sum_k c_k dE/dc_k = 0 per orbital and central differences on sampled weights.
 """
 import json
@@ -98,6 +98,9 @@ def test_lif128_inverse_residual(lib, lif128):
 
 
 def _tol(e):
+    """Cross-path budget (J/K pass against energy pass, different kernels and
+    summation orders): 1e-13 of |E|, 2.4e-8 Eh at LiF128's 2.4e5 Eh.  The
+    invariance tests of one path hold the 1e-9 Eh gate itself."""
     return 1e-9 * max(1.0, abs(e) / 1e4)
 
 
@@ -119,7 +122,7 @@ def test_lif128_invariances(lib, lif128):
         assert int(st[0]) + int(st[2]) == _unique_quartets(s)
         # repeated calls (the 384 x 384 overlap inverse runs one pivot per pair of grid-wide launches)
         for _ in range(3):
-            assert abs(ctx.energy() - e0) < _tol(e0)
+            assert abs(ctx.energy() - e0) < 1e-9
         # scale 40 random orbitals
         c = s.coeff.copy()
         for p in rng.choice(s.norbs, 40, replace=False):
@@ -139,10 +142,10 @@ def test_lif128_invariances(lib, lif128):
         print(f"  mixed orbitals: dE={e2 - e0:.3e}", flush=True)
         # rigid translation
         e3 = ctx.energy(s.coeff, s.coords + np.array([0.31, -0.17, 0.73]))
-    print(f"dE scale={e1 - e0:.3e} mix={e2 - e0:.3e} translate={e3 - e0:.3e} tol={_tol(e0):.1e}")
-    assert abs(e1 - e0) < _tol(e0)
-    assert abs(e2 - e0) < _tol(e0)
-    assert abs(e3 - e0) < _tol(e0)
+    print(f"dE scale={e1 - e0:.3e} mix={e2 - e0:.3e} translate={e3 - e0:.3e} (gate 1e-9 Eh)")
+    assert abs(e1 - e0) < 1e-9
+    assert abs(e2 - e0) < 1e-9
+    assert abs(e3 - e0) < 1e-9
 
 
 def test_lif128_gradient(lib, lif128):
@@ -164,6 +167,7 @@ def test_lif128_gradient(lib, lif128):
             cm = s.coeff.copy(); cm[k] -= h
             fd[n] = (ctx.energy(cp) - ctx.energy(cm)) / (2 * h)
             print(f"  weight {k}: grad {g[k]:.10e} fd {fd[n]:.10e}", flush=True)
+    print(f"energy of the J/K pass - energy pass: {e - e0:.3e} Eh (budget {_tol(e0):.1e})")
     assert abs(e - e0) < _tol(e0)
     rel = np.linalg.norm(g[idx] - fd) / np.linalg.norm(fd)
     print(f"E={e:.10f} |g|={np.linalg.norm(g):.4e} sampled rel err vs FD={rel:.3e}")
