@@ -28,13 +28,14 @@ constexpr int kBoysK = kLTot + 9;
 // Compact tables, one per L = 0..kLTot, for the straight-line quartet kernels
 // (staged in LDS): c_k(T_i) = F_{L+k}(T_i) / k!, T_i = i * kBoysCDT, k < kBoysCK.
 // F_L(T) = sum_k c_k d^k and F_{L+1}(T) = its derivative in d, d = T_i - T,
-// |d| <= 0.1: truncation below 3e-19 F_{L+12}(T_i).  Rows are padded to 14
-// doubles: with 12 the 64 lanes' random rows start on only 4 distinct LDS bank
-// groups (measured: the C8H18 J/K pass 6.99 -> 6.84 ms with the padding).
+// |d| <= 0.1: the first omitted terms are below 2.5e-19 F_{L+11}(T_i) (value)
+// and 2.8e-17 F_{L+11}(T_i) (derivative).  Rows are padded to 14 doubles: with
+// 12 the 64 lanes' random rows start on only 4 distinct LDS bank groups
+// (measured: the C8H18 J/K pass 6.99 -> 6.84 ms with the padding).
 constexpr double kBoysCDT = 0.2;
 constexpr double kBoysCInvDT = 5.0;
 constexpr int kBoysCN = 301;  // T_i = 0 .. 60
-constexpr int kBoysCK = 12;
+constexpr int kBoysCK = 11;
 constexpr int kBoysCRow = 14;  // row stride (doubles)
 constexpr int kBoysCLen = kBoysCN * kBoysCRow;  // doubles per L table
 constexpr size_t kBoysCOff = (size_t)kBoysN * kBoysK;  // compact tables follow the full one
@@ -139,9 +140,12 @@ __device__ __forceinline__ void boys_eval_c(double T, const double* __restrict__
     const int i = __double2int_rn(T * kBoysCInvDT);
     const double d = i * kBoysCDT - T;
     const double* c = tab + i * kBoysCRow;
-    double f = c[kBoysCK - 1], g = 0.0;
+    // Horner for the polynomial (f) and, one step behind, its derivative (g)
+    double f = c[kBoysCK - 1];
+    double g = f;
+    f = __builtin_fma(f, d, c[kBoysCK - 2]);
 #pragma unroll
-    for (int k = kBoysCK - 2; k >= 0; --k) {
+    for (int k = kBoysCK - 3; k >= 0; --k) {
       if constexpr (L > 0) g = __builtin_fma(g, d, f);
       f = __builtin_fma(f, d, c[k]);
     }
