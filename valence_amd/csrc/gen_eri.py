@@ -327,9 +327,9 @@ def gen_class(la, lb, lc, ld, uniform_bra=False):
     blk.emit("        const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
     blk.emit("        const double pref = B.K * K.K * r;")
     blk.emit(f"        double Fb[{L + 1}];")
-    blk.emit(f"        boys_eval_c<{L}>(T, btab, Fb);")
+    blk.emit(f"        boys_eval_c<{L}>(T, pref, btab, Fb);  // pref * F_m(T)")
     for m in range(L + 1):
-        blk.emit(f"        const double F{m} = pref * Fb[{m}];")
+        blk.emit(f"        const double F{m} = Fb[{m}];")
     if la + lb > 0:
         blk.emit("        const double WPx = -K.p * rs * PQx, WPy = -K.p * rs * PQy, WPz = -K.p * rs * PQz;")
         blk.emit("        const double rop = K.p * rs;")

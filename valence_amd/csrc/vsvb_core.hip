@@ -2441,7 +2441,7 @@ __global__ void k_boys_probe(const double* __restrict__ T, int n, const double* 
   if (i >= n) return;
   double F[L + 1];
   if (compact)
-    boys_eval_c<L>(T[i], tab + kBoysCOff + (size_t)L * kBoysCLen, F);
+    boys_eval_c<L>(T[i], 1.0, tab + kBoysCOff + (size_t)L * kBoysCLen, F);
   else
     boys_eval<L>(T[i], tab, F);
 #pragma unroll

@@ -134,11 +134,15 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
 
 // Boys function from the compact table of this L (tab = its first row): one
 // polynomial in d for F_L and its derivative for F_{L+1}, then as boys_eval.
+// Every F_m comes out multiplied by w (the primitive quartet's prefactor): the
+// recursions are linear, so scaling their two starting values (or F_0 of the
+// asymptotic branch) scales them all.
 template <int L>
-__device__ __forceinline__ void boys_eval_c(double T, const double* __restrict__ tab, double* F) {
+__device__ __forceinline__ void boys_eval_c(double T, double w, const double* __restrict__ tab, double* F) {
   if (T < kBoysTMax) {
-    const int i = __double2int_rn(T * kBoysCInvDT);
-    const double d = i * kBoysCDT - T;
+    const double x = __builtin_rint(T * kBoysCInvDT);
+    const int i = (int)x;
+    const double d = __builtin_fma(x, kBoysCDT, -T);
     const double* c = tab + i * kBoysCRow;
     // Horner for the polynomial (f) and, one step behind, its derivative (g)
     double f = c[kBoysCK - 1];
@@ -149,8 +153,10 @@ __device__ __forceinline__ void boys_eval_c(double T, const double* __restrict__
       if constexpr (L > 0) g = __builtin_fma(g, d, f);
       f = __builtin_fma(f, d, c[k]);
     }
+    f *= w;
     F[L] = f;
     if constexpr (L > 0) {
+      g *= w;
       const double t2 = 2.0 * T;
       const double e = __builtin_fma(-t2, g, (2 * L + 1) * f);
 #pragma unroll
@@ -158,7 +164,7 @@ __device__ __forceinline__ void boys_eval_c(double T, const double* __restrict__
     }
   } else {
     const double r = rsqrt_fp64(T);
-    F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
+    F[0] = (0.88622692545275801365 * w) * r;  // sqrt(pi)/2
     if constexpr (L > 0) {
       const double h = 0.5 * r * r;
 #pragma unroll
