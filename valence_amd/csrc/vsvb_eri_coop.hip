@@ -192,30 +192,71 @@ __device__ __forceinline__ void coop_body(const QuartetArgs& a, const CoopDesc& 
           for (int ww = 0; ww < kMaxW; ++ww)
             if (ww == w) esum[ww] += deg * e;
         } else {
+          // The contracted block V (contiguous, [ab][cd]) and the density blocks
+          // the contraction reads go to LDS first: the loops below then read LDS
+          // only, instead of a step-table index and a global density element per
+          // term.  V and the blocks follow the kernel's VRR/HRR slots (coop_lds_bytes).
+          double* V = S + d.lds;
+          double* Rab = V + NAB * NCD;  // cfac * sum_s D_s[a][b], then [c][d]
+          double* Rcd = Rab + NAB;
+          double* Dbd = Rcd + NCD;      // one spin's D[b][d], D[a][d], D[b][c], D[a][c]
+          double* Dad = Dbd + NB * ND;
+          double* Dbc = Dad + NA * ND;
+          double* Dac = Dbc + NB * NC;
+          if (w == 0)
+            for (int idx = lane; idx < NAB * NCD; idx += 64) V[idx] = S[outs[idx]];
+          __syncthreads();  // V is written; the previous density's blocks are consumed
+          for (int x = lane; x < NAB + NCD; x += 64) {
+            if (x < NAB)
+              Rab[x] = rho(ma + x / NB, mb + x % NB);
+            else
+              Rcd[x - NAB] = rho(mc + (x - NAB) / ND, md + (x - NAB) % ND);
+          }
+          __syncthreads();
           const double fj = 0.25 * deg, fk = 0.125 * deg * xfac;
           double* J = a.jacc + (size_t)w * nn;
           for (int ab = lane; ab < NAB; ab += 64) {
             const int i = ab / NB, j = ab - i * NB;
             double t = 0.0;
-            for (int cd = 0; cd < NCD; ++cd) t += S[outs[ab * NCD + cd]] * rho(mc + cd / ND, md + cd % ND);
+            for (int cd = 0; cd < NCD; ++cd) t += V[ab * NCD + cd] * Rcd[cd];
             unsafeAtomicAdd(&J[(size_t)(ma + i) * n + mb + j], fj * t);
           }
           for (int cd = lane; cd < NCD; cd += 64) {
             const int k = cd / ND, l = cd - k * ND;
             double t = 0.0;
-            for (int ab = 0; ab < NAB; ++ab) t += S[outs[ab * NCD + cd]] * rho(ma + ab / NB, mb + ab % NB);
+            for (int ab = 0; ab < NAB; ++ab) t += V[ab * NCD + cd] * Rab[ab];
             unsafeAtomicAdd(&J[(size_t)(mc + k) * n + md + l], fj * t);
           }
           for (int s = 0; s < ns; ++s) {
             const double* Ds = D0 + s * nn;
             double* Kx = a.kacc + ((size_t)w * ns + s) * nn;
+            if (s > 0) __syncthreads();  // the previous spin's blocks are consumed
+            for (int x = lane; x < NB * ND + NA * ND + NB * NC + NA * NC; x += 64) {
+              int y = x;
+              if (y < NB * ND) {
+                Dbd[y] = Ds[(size_t)(mb + y / ND) * n + md + y % ND];
+                continue;
+              }
+              y -= NB * ND;
+              if (y < NA * ND) {
+                Dad[y] = Ds[(size_t)(ma + y / ND) * n + md + y % ND];
+                continue;
+              }
+              y -= NA * ND;
+              if (y < NB * NC) {
+                Dbc[y] = Ds[(size_t)(mb + y / NC) * n + mc + y % NC];
+                continue;
+              }
+              y -= NB * NC;
+              Dac[y] = Ds[(size_t)(ma + y / NC) * n + mc + y % NC];
+            }
+            __syncthreads();
             // B[a][c] += v D[b][d]
             for (int x = lane; x < NA * NC; x += 64) {
               const int i = x / NC, k = x - i * NC;
               double t = 0.0;
               for (int j = 0; j < NB; ++j)
-                for (int l = 0; l < ND; ++l)
-                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + md + l];
+                for (int l = 0; l < ND; ++l) t += V[(i * NB + j) * NCD + k * ND + l] * Dbd[j * ND + l];
               unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + mc + k], fk * t);
             }
             // B[b][c] += v D[a][d]
@@ -223,8 +264,7 @@ __device__ __forceinline__ void coop_body(const QuartetArgs& a, const CoopDesc& 
               const int j = x / NC, k = x - j * NC;
               double t = 0.0;
               for (int i = 0; i < NA; ++i)
-                for (int l = 0; l < ND; ++l)
-                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + md + l];
+                for (int l = 0; l < ND; ++l) t += V[(i * NB + j) * NCD + k * ND + l] * Dad[i * ND + l];
               unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + mc + k], fk * t);
             }
             // B[a][d] += v D[b][c]
@@ -232,8 +272,7 @@ __device__ __forceinline__ void coop_body(const QuartetArgs& a, const CoopDesc& 
               const int i = x / ND, l = x - i * ND;
               double t = 0.0;
               for (int j = 0; j < NB; ++j)
-                for (int k = 0; k < NC; ++k)
-                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(mb + j) * n + mc + k];
+                for (int k = 0; k < NC; ++k) t += V[(i * NB + j) * NCD + k * ND + l] * Dbc[j * NC + k];
               unsafeAtomicAdd(&Kx[(size_t)(ma + i) * n + md + l], fk * t);
             }
             // B[b][d] += v D[a][c]
@@ -241,8 +280,7 @@ __device__ __forceinline__ void coop_body(const QuartetArgs& a, const CoopDesc& 
               const int j = x / ND, l = x - j * ND;
               double t = 0.0;
               for (int i = 0; i < NA; ++i)
-                for (int k = 0; k < NC; ++k)
-                  t += S[outs[(i * NB + j) * NCD + k * ND + l]] * Ds[(size_t)(ma + i) * n + mc + k];
+                for (int k = 0; k < NC; ++k) t += V[(i * NB + j) * NCD + k * ND + l] * Dac[i * NC + k];
               unsafeAtomicAdd(&Kx[(size_t)(mb + j) * n + md + l], fk * t);
             }
           }
@@ -308,13 +346,24 @@ int coop_class(int la, int lb, int lc, int ld) {
   return -1;
 }
 
-size_t coop_lds_bytes(int idx) { return (size_t)kCoopDesc[idx].lds * sizeof(double); }
+// LDS of one workgroup: the VRR/HRR slots and, in J/K mode, the contiguous
+// contracted block, the two Coulomb density blocks and one spin's four
+// exchange density blocks.
+size_t coop_lds_bytes(int idx, bool jk) {
+  const CoopDesc& d = kCoopDesc[idx];
+  size_t n = (size_t)d.lds;
+  if (jk) {
+    const size_t NA = ncart(d.la), NB = ncart(d.lb), NC = ncart(d.lc), ND = ncart(d.ld);
+    n += NA * NB * NC * ND + NA * NB + NC * ND + NB * ND + NA * ND + NB * NC + NA * NC;
+  }
+  return n * sizeof(double);
+}
 
 bool coop_default(int idx) { return idx >= 0 && kCoopDesc[idx].dflt != 0; }
 
 int coop_occupancy(int idx, bool jk) {
   int nb = 0;
-  const size_t lds = coop_lds_bytes(idx);
+  const size_t lds = coop_lds_bytes(idx, jk);
   if (jk)
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, coop_kernel<true>, 64, lds);
   else
@@ -347,7 +396,7 @@ hipError_t launch_coop(int idx, const QuartetArgs& a, bool jk, int nblocks, int 
   const int* t = nullptr;
   hipError_t e = coop_tables(device, &t);
   if (e != hipSuccess) return e;
-  const size_t lds = coop_lds_bytes(idx);
+  const size_t lds = coop_lds_bytes(idx, jk);
   if (jk)
     hipLaunchKernelGGL(coop_kernel<true>, dim3(nblocks), dim3(64), lds, st, a, kCoopDesc[idx], t);
   else
@@ -361,7 +410,7 @@ hipError_t launch_coop_groups(int idx, const QuartetArgs* gs, int ng, int nblock
   const int* t = nullptr;
   hipError_t e = coop_tables(device, &t);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(coop_groups_kernel, dim3(nblocks), dim3(64), coop_lds_bytes(idx), st, gs, ng, kCoopDesc[idx], t,
+  hipLaunchKernelGGL(coop_groups_kernel, dim3(nblocks), dim3(64), coop_lds_bytes(idx, true), st, gs, ng, kCoopDesc[idx], t,
                      stamp);
   return hipGetLastError();
 }
