@@ -85,7 +85,7 @@ def tuner_summary(t) -> dict:
             if r[7] <= 0:
                 continue
             if r[0]:
-                kind = f"coop claim={int(r[3]) or 'auto'}"
+                kind = f"coop claim={int(r[3]) or 'auto'} wg/cu={int(r[1]) or 'occ'}"
             else:
                 kind = f"{'transposed ' if r[4] else ''}{'lds' if r[2] else 'atomic'} wg/cu={int(r[1]) or 'occ'}"
             d[CLASS_NAMES[q]] = f"{kind} {r[5]:.3f} ms (default {r[6]:.3f})"

@@ -2174,7 +2174,8 @@ static int batch_gather(vsvb_batch* b) {
 // own data: the straight-line kernel with 1, 2, 4 or as many workgroups per CU
 // as fit, with or without the LDS exchange staging, in both orientations
 // (classes with different bra and ket classes), and the cooperative kernel
-// (classes with step tables) with automatic, 1-, 8- or 64-unit claims.  The
+// (classes with step tables) with automatic, 1-, 8- or 64-unit claims and as
+// many, 8, 4 or 2 workgroups per CU.  The
 // fastest (best of two launches) becomes the class policy of every group of
 // the bucket (a candidate already 1.5x slower than the best after one launch
 // gets no second).  The launches accumulate into the leader's J/K arrays,
@@ -2244,12 +2245,14 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
         cand.push_back(p);
       }
       if (L->coop_idx[q] >= 0)
-        for (int cl : {0, 1, 8, 64}) {
-          Policy p;
-          p.coop = true;
-          p.claim = cl;
-          cand.push_back(p);
-        }
+        for (int pc : {0, 8, 4, 2})
+          for (int cl : {0, 1, 8, 64}) {
+            Policy p;
+            p.coop = true;
+            p.per_cu = pc;
+            p.claim = cl;
+            cand.push_back(p);
+          }
       double tbest = 1e30, tdef = 0.0;
       for (size_t k = 0; k < cand.size(); ++k) {
         const ClassLaunch cl = policy_launch(L, q, cand[k]);
