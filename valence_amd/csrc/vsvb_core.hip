@@ -2175,11 +2175,15 @@ static int batch_gather(vsvb_batch* b) {
 // apart: for each bucket a representative closed- or high-spin group leader --
 // single: the molecule size that occurs most often; many: the group with the
 // most density work -- times every candidate configuration of every class on its
-// own data: the straight-line kernel with 1, 2, 4 or as many workgroups per CU
-// as fit, with or without the LDS exchange staging, in both orientations
-// (classes with different bra and ket classes), and the cooperative kernel
-// (classes with step tables) with automatic, 1-, 8- or 64-unit claims and as
-// many, 8, 4 or 2 workgroups per CU.  The
+// own data: the straight-line kernel with 1 or 2 workgroups per CU, with or
+// without the LDS exchange staging, in both orientations (classes with
+// different bra and ket classes), and the cooperative kernel (classes with step
+// tables) with automatic, 1-, 8- or 64-unit claims and as many, 8, 4 or 2
+// workgroups per CU.  Straight-line launches stay at 1-2 workgroups per CU: a
+// class timed alone ranks 4 or as many as fit first about as often, but in the
+// batch four lane streams run classes side by side, and capping the straight
+// launches at 2 per CU measured 330-333 -> 339-347 evals/s on the 96-member
+// C8H18 batch (1 per CU: 331-338).  The
 // fastest (best of two launches) becomes the class policy of every group of
 // the bucket (a candidate already 1.5x slower than the best after one launch
 // gets no second).  The launches accumulate into the leader's J/K arrays,
@@ -2228,10 +2232,10 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       Policy d0;  // the default
       d0.coop = L->ljk[q].coop;
       d0.tr = L->ljk[q].tr;
-      d0.per_cu = L->ljk[q].per_cu;
+      d0.per_cu = L->ljk[q].coop ? L->ljk[q].per_cu : 2;
       cand.push_back(d0);
       const bool st_n = straight_allowed(L, q, true, false), st_t = bc != kc && straight_allowed(L, q, true, true);
-      for (int pc : {0, 1, 2, 4})
+      for (int pc : {1, 2})
         for (bool lds : {true, false}) {
           Policy p;
           p.per_cu = pc;
@@ -2246,6 +2250,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
         Policy p;
         p.tr = true;
         p.lds_on = false;
+        p.per_cu = 2;
         cand.push_back(p);
       }
       if (L->coop_idx[q] >= 0)
