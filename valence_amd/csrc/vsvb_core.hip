@@ -2178,12 +2178,14 @@ static int batch_gather(vsvb_batch* b) {
 // own data: the straight-line kernel with 1 or 2 workgroups per CU, with or
 // without the LDS exchange staging, in both orientations (classes with
 // different bra and ket classes), and the cooperative kernel (classes with step
-// tables) with automatic, 1-, 8- or 64-unit claims and as many, 8, 4 or 2
-// workgroups per CU.  Straight-line launches stay at 1-2 workgroups per CU: a
-// class timed alone ranks 4 or as many as fit first about as often, but in the
-// batch four lane streams run classes side by side, and capping the straight
-// launches at 2 per CU measured 330-333 -> 339-347 evals/s on the 96-member
-// C8H18 batch (1 per CU: 331-338).  The
+// tables) with automatic, 1-, 8- or 64-unit claims and 8, 4 or 2 workgroups
+// per CU.  Both kernels stay below their occupancy: a class timed alone often
+// ranks 4 or as many straight-line workgroups per CU as fit first, and as many
+// cooperative ones as fit, but in the batch four lane streams run classes side
+// by side.  On the 96-member C8H18 batch, capping the straight-line launches at
+// 2 per CU measured 330-333 -> 339-347 evals/s (1 per CU: 331-338), then
+// dropping the cooperative kernel's as-many-as-fit candidate 340 -> 349-358
+// (cooperative candidates 4 and 2 only: 340-341).  The
 // fastest (best of two launches) becomes the class policy of every group of
 // the bucket (a candidate already 1.5x slower than the best after one launch
 // gets no second).  The launches accumulate into the leader's J/K arrays,
@@ -2232,7 +2234,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
       Policy d0;  // the default
       d0.coop = L->ljk[q].coop;
       d0.tr = L->ljk[q].tr;
-      d0.per_cu = L->ljk[q].coop ? L->ljk[q].per_cu : 2;
+      d0.per_cu = L->ljk[q].coop ? 8 : 2;
       cand.push_back(d0);
       const bool st_n = straight_allowed(L, q, true, false), st_t = bc != kc && straight_allowed(L, q, true, true);
       for (int pc : {1, 2})
@@ -2254,7 +2256,7 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
         cand.push_back(p);
       }
       if (L->coop_idx[q] >= 0)
-        for (int pc : {0, 8, 4, 2})
+        for (int pc : {8, 4, 2})
           for (int cl : {0, 1, 8, 64}) {
             Policy p;
             p.coop = true;
