@@ -1569,20 +1569,25 @@ int vsvb::run_quartets(vsvb_context* c, bool jk, double* e2_dev, int nwfn, const
   const int nslot = 2 * kStampSlots * kNQuartetClass;
   if (c->profile) hipLaunchKernelGGL(k_stamp_reset, dim3((nslot + 255) / 256), dim3(256), 0, st, c->d_stamps.p, nslot);
   if (c->ev0 && !c->capturing) HIP_OK(hipEventRecord(c->ev0, st));
-  size_t poff = 0;
-  for (int bc = 0; bc < kNPairClass; ++bc)
-    for (int kc = 0; kc <= bc; ++kc) {
-      const int q = quartet_class_index(bc, kc);
-      const ClassLaunch& cl = jk ? c->ljk[q] : c->le[q];
-      const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
-      // a class with pairs on both sides always has work units: a launch
-      // configuration without any would drop its quartets silently
-      if (nb > 0 && nk > 0 && cl.units <= 0)
-        return fail(VSVB_ERR_UNSUPPORTED, "quartet class " + std::to_string(q) + " has no launchable work units");
-      const QuartetArgs a = class_args(c, bc, kc, cl, jk, nwfn, nspin, dens, jacc, kacc, c->d_partial.p + poff);
-      HIP_OK(launch_class(c, bc, kc, cl, a, jk));
-      poff += c->partial_per_class[q];
-    }
+  // Classes in class order, or reversed (c->class_rev); each class's partial
+  // sums keep their own offset either way.
+  size_t offs[kNQuartetClass], poff = 0;
+  for (int q = 0; q < kNQuartetClass; ++q) {
+    offs[q] = poff;
+    poff += c->partial_per_class[q];
+  }
+  for (int t = 0; t < kNQuartetClass; ++t) {
+    const int q = c->class_rev ? kNQuartetClass - 1 - t : t;
+    const int bc = class_bra(q), kc = q - quartet_class_index(bc, 0);
+    const ClassLaunch& cl = jk ? c->ljk[q] : c->le[q];
+    const int nb = c->pair_off[bc + 1] - c->pair_off[bc], nk = c->pair_off[kc + 1] - c->pair_off[kc];
+    // a class with pairs on both sides always has work units: a launch
+    // configuration without any would drop its quartets silently
+    if (nb > 0 && nk > 0 && cl.units <= 0)
+      return fail(VSVB_ERR_UNSUPPORTED, "quartet class " + std::to_string(q) + " has no launchable work units");
+    const QuartetArgs a = class_args(c, bc, kc, cl, jk, nwfn, nspin, dens, jacc, kacc, c->d_partial.p + offs[q]);
+    HIP_OK(launch_class(c, bc, kc, cl, a, jk));
+  }
   if (c->ev0 && c->ev1 && !c->capturing) {
     HIP_OK(hipEventRecord(c->ev1, st));
     c->ev_valid = true;
@@ -2099,8 +2104,13 @@ static int group_gradients(vsvb_batch* b, BatchGroup& g) {
 static int batch_enqueue_groups(vsvb_batch* b) {
   int rc;
   if (!b->planned || b->chunks <= 0) {
+    // Groups on odd lanes run their classes in reverse order, so concurrent
+    // lanes tend to pair the VALU-bound low-angular-momentum classes with the
+    // latency-bound cooperative (d-heavy) ones.  Measured on the 96-member
+    // C8H18 batch over four runs each: 360-362 evals/s against 350-361.
     for (int gi : b->order) {
       BatchGroup& g = b->groups[gi];
+      b->ctx[g.members[0]]->class_rev = b->lane_of[gi] & 1;
       if ((rc = group_front(b, g, true))) return rc;
       if ((rc = group_gradients(b, g))) return rc;
     }

@@ -198,6 +198,7 @@ struct vsvb_context {
   bool last_jk = false;
   int last_nwfn = 1, last_nspin = 1;  // densities and spin blocks of the last quartet launch
   bool ran_quartets = false;  // this context launched the quartet stage in the current call
+  int class_rev = 0;          // quartet classes launched in reverse class order (odd batch lanes)
   bool capturing = false;     // enqueueing into a HIP graph capture: no host uploads
   double class_stats[vsvb::kNQuartetClass][6] = {};
   // spin-coupled wave functions (npair > 0): determinant lists and work arrays
