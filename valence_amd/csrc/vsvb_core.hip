@@ -2197,8 +2197,9 @@ static int batch_gather(vsvb_batch* b) {
 // dropping the cooperative kernel's as-many-as-fit candidate 340 -> 349-358
 // (cooperative candidates 4 and 2 only: 340-341).  The
 // fastest (best of two launches) becomes the class policy of every group of
-// the bucket (a candidate already 1.5x slower than the best after one launch
-// gets no second).  The launches accumulate into the leader's J/K arrays,
+// the bucket if it beats the default (the first candidate) by 3% or more (a
+// candidate already 1.5x slower than the best after one launch gets no
+// second).  The launches accumulate into the leader's J/K arrays,
 // which the first run clears again.
 struct Policy {
   bool coop = false;
@@ -2296,6 +2297,15 @@ static int tune_leader(vsvb_batch* b, BatchGroup* gs, double (*tuning)[kTuneCols
           tbest = tmin;
           best[q] = cand[k];
         }
+      }
+      // A candidate replaces the default only if it is at least 3% faster:
+      // closer races resolved differently from process to process, and some
+      // of those picks overlapped worse in the batch (six batch runs each on
+      // the 96-member C8H18 batch: 352-361 evals/s, one run in six at 352,
+      // against 359-363 with the margin).
+      if (tdef > 0.0 && tbest > 0.97 * tdef) {
+        best[q] = cand[0];
+        tbest = tdef;
       }
       double* t = tuning[q];
       t[0] = best[q].coop ? 1 : 0;
