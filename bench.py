@@ -142,8 +142,8 @@ def main():
                          "the 128-atom LiF slab")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU; the path shards by "
-              "whole wave functions across GPUs (see DESIGN.md) and no multi-GPU figure is produced.",
+        print(f"bench.py: --gpus {args.gpus} is not supported: one GPU is measured; the path would shard by "
+              "whole wave functions (batch groups) across GPUs, and no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
 

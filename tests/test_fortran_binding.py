@@ -2,7 +2,7 @@
 valence_amd/fortran/vsvb_hip.F90 and a driver program are built with amdflang
 and linked against libvsvb_hip.so, and the bind(C) types must lay out exactly
 as include/vsvb_hip.h does under the C compiler (offsets from a gcc probe).
-Running the driver on the GPU is tests/test_gpu_fortran.py."""
+Running that program on the GPU is tests/test_gpu_fortran.py."""
 import os
 import shutil
 import subprocess
