@@ -239,7 +239,9 @@ def test_batch_graph_off_matches(lib):
 
 def test_batch_lanes_match(lib):
     """Groups overlap on several streams by default; one or two streams must give
-    the same results (each group writes only its own buffers)."""
+    the same results (each group writes only its own buffers).  Groups on odd
+    streams launch their classes in reverse order, so this also compares the
+    two class orders."""
     systems = workloads.config3()
     pick = [2, 3, 51, 52, 93, 94]
     b = lib.Batch([systems[i] for i in pick])
